@@ -1,0 +1,150 @@
+"""Benchmark of the fiber3d GMRES-matvec hot path on one MI355X.
+
+Workload (one "step" = one pass of the dominant hot-path kernel over one
+batch): the raw Oseen source-target sum with N = M = 65536 points uniform in
+the unit cube, forces N(0,1), viscosity 1, self term excluded, evaluated with
+the FP32-compensated kernel (FP64 output).  Inputs are resident in HBM before
+the timed region.  value = Gpair-interactions/s over the K timed steps.
+
+Also reported on the same line: the FP64 kernel's rate and its error, the
+stresslet rates, the roofline of the dominant kernel (FP32 VALU, 28 flops per
+Oseen pair) and a CPU baseline (the oracle's C restatement of the reference's
+numba loop, OpenMP over targets, on a bounded sample of targets).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "time-steps/s at 1e-8 GMRES residual, learned+FP32 vs plain FP64; Gpairs/s"
+N_PTS = 65536
+OSEEN_FLOPS = 28      # per pair: 3 sub, r^2 5, rsqrt 1, d.f 5, r^-2 1, scale 1, 3 x 2 FMA
+STRESSLET_FLOPS = 30
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak (packed)
+PEAK_FP64_TFLOPS = 78.6
+
+
+def cpu_baseline(x, f, n_targets=4096, cores=None):
+    """Oracle C restatement of kernels.py:794-846 on the host (port)."""
+    from oracle import build as obuild
+    from oracle import cbaseline
+    if not os.path.exists(obuild.OUT):
+        obuild.build(verbose=False)
+    cores = cores or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    rows = np.arange(n_targets)
+    cbaseline.oseen(x, x[:64], f)  # warm-up (thread pool)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        cbaseline.oseen(x, x[rows], f)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    return {"value": round(N_PTS * n_targets / dt / 1e9, 4), "unit": "Gpairs/s", "cores": cores,
+            "kind": "port",
+            "sample": f"Oseen FP64, all {N_PTS} sources x {n_targets} targets, median of 3"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
+              "the multi-GPU sharding is a design in DESIGN.md, not a measured path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fiber3d_amd import kernels as FK
+
+    dev = FK.device()
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0.0, 1.0, size=(N_PTS, 3))
+    f = rng.normal(size=(N_PTS, 3))
+    nrm = rng.normal(size=(N_PTS, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    mu = rng.normal(size=(N_PTS, 3))
+    wq = rng.uniform(0.5, 1.5, size=N_PTS) / N_PTS
+    X = torch.from_numpy(x).to(dev)
+    F = torch.from_numpy(f).to(dev)
+    NRM = torch.from_numpy(nrm).to(dev)
+    Q = torch.from_numpy(mu * wq[:, None]).to(dev)
+    idx = torch.arange(N_PTS, dtype=torch.int32, device=dev)
+    excl = (idx, idx + 1)
+    out = torch.empty((N_PTS, 3), dtype=torch.float64, device=dev)
+    pairs = float(N_PTS) * N_PTS
+
+    def run(kernel, precision, steps, warmup, nrm_t=None, den=F):
+        for _ in range(warmup):
+            FK.pairsum(kernel, X, den, X, src_nrm=nrm_t, excl=excl, precision=precision, out=out)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
+        t0 = time.perf_counter()
+        for s, e in ev:
+            s.record(stream)
+            FK.pairsum(kernel, X, den, X, src_nrm=nrm_t, excl=excl, precision=precision, out=out)
+            e.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms = float(np.mean([s.elapsed_time(e) for s, e in ev]))
+        return wall / steps * 1e3, kms, out.clone()
+
+    # headline: FP32-compensated Oseen
+    ms32, k32, u32 = run("oseen", "f32c", args.steps, args.warmup)
+    ms64, k64, u64 = run("oseen", "f64", max(3, args.steps // 4), 1)
+    err32 = float(torch.linalg.norm(u32 - u64) / torch.linalg.norm(u64))
+    sms32, sk32, s32 = run("stresslet", "f32c", max(3, args.steps // 4), 1, nrm_t=NRM, den=Q)
+    sms64, sk64, s64 = run("stresslet", "f64", max(3, args.steps // 4), 1, nrm_t=NRM, den=Q)
+    serr = float(torch.linalg.norm(s32 - s64) / torch.linalg.norm(s64))
+
+    gp = pairs / (ms32 / 1e3) / 1e9
+    achieved = OSEEN_FLOPS * pairs / (k32 / 1e3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": round(gp, 3),
+        "unit": "Gpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (compensated; FP64 output)",
+        "data": "synthetic (seeded: 65536 points U[0,1)^3, forces N(0,1))",
+        "config": {"workload": "raw Oseen source-target sum, N=M=65536, self term excluded, "
+                               "FP32-compensated", "n_sources": N_PTS, "n_targets": N_PTS,
+                   "eta": 1.0},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "kernel": "pairsum_kernel<float,OSEEN>", "flops_per_pair": OSEEN_FLOPS,
+                     "kernel_ms": round(k32, 4)},
+        "oseen_f32c_rel_err_vs_f64": err32,
+        "oseen_f64_gpairs_s": round(pairs / (ms64 / 1e3) / 1e9, 3),
+        "oseen_f64_tflops": round(OSEEN_FLOPS * pairs / (k64 / 1e3) / 1e12, 3),
+        "stresslet_f32c_gpairs_s": round(pairs / (sms32 / 1e3) / 1e9, 3),
+        "stresslet_f64_gpairs_s": round(pairs / (sms64 / 1e3) / 1e9, 3),
+        "stresslet_f32c_rel_err_vs_f64": serr,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(x, f)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,76 @@
+"""Build libf3d.so (every HIP source under csrc/) for gfx950, in-tree.
+
+Run as ``python -m fiber3d_amd.build``.  The library lands next to this file
+so that it travels with the source tree; it is not installed anywhere else.
+"""
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libf3d.so")
+ARCH = os.environ.get("F3D_OFFLOAD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    # Packed FP32 VALU forms are kept out of every kernel.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _hipcc():
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cand = os.path.join(rocm, "bin", "hipcc")
+    return cand if os.path.exists(cand) else "hipcc"
+
+
+def sources():
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+
+
+def build(verbose: bool = True, jobs: int = 8) -> str:
+    """Compile every csrc/*.hip to an object and link libf3d.so."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = sources()
+    t0 = time.time()
+    procs = []
+    objs = []
+    for s in srcs:
+        o = os.path.join(objdir, os.path.basename(s).replace(".hip", ".o"))
+        objs.append(o)
+        cmd = [_hipcc(), *HIPCC_FLAGS, "-c", s, "-o", o]
+        procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        while sum(p.poll() is None for _, p in procs) >= jobs:
+            time.sleep(0.05)
+    failed = False
+    for s, p in procs:
+        out = p.communicate()[0].decode(errors="replace")
+        # the host pass does not know the packed-fp32 feature and says so
+        lines = [l for l in out.splitlines() if "packed-fp32-ops" not in l]
+        if p.returncode != 0:
+            failed = True
+            sys.stderr.write(f"[f3d build] {os.path.basename(s)} failed:\n" + "\n".join(lines) + "\n")
+        elif verbose and lines:
+            sys.stderr.write("\n".join(lines) + "\n")
+    if failed:
+        raise RuntimeError("libf3d build failed")
+    tmp = LIB + ".tmp"
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("libf3d link failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"[f3d build] {LIB} from {len(srcs)} sources in {time.time() - t0:.1f}s")
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

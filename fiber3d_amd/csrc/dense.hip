@@ -1,0 +1,589 @@
+// Batched dense kernels over per-fiber blocks of uneven order (n <= 256):
+//   * LU factorisation with partial pivoting (LAPACK dgetrf semantics), FP64
+//     or equilibrated FP32, one persistent 1024-thread workgroup per block;
+//   * forward/backward substitution with optional FP64 iterative refinement,
+//     one wavefront per block;
+//   * column-major block GEMV, one wavefront per block.
+//
+// Reference: scipy.linalg.lu_factor / lu_solve per fiber in
+// python/tstep/tstep_utils.py:1405-1419 and python/tstep/tstep.py:985-988;
+// the block-diagonal apply As_fibers.dot(XT) in python/tstep/tstep.py:1173.
+//
+// LU layout on the CU: the 1024 threads form a 32 x 32 grid (tr = row lane,
+// tc = column lane); thread (tr, tc) owns rows tr + 32a and columns tc + 32b,
+// a, b in [0, 8), i.e. a 2-D cyclic distribution that keeps every thread busy
+// as the trailing matrix shrinks.  FP32 blocks (256 KB at n = 256) live wholly
+// in registers.  FP64 blocks (512 KB) do not fit the register file, so the
+// last 64 columns live in LDS (128 KB) and the rest in registers.  Each
+// elimination step costs two workgroup barriers: pivot search (column owners,
+// half-wave shuffles) -> publish pivot row / old row k / column k through LDS
+// -> every thread swaps, forms its multipliers and updates its tile.
+#include "common.h"
+
+namespace f3d {
+
+constexpr int kLuG = 32;     // thread grid is kLuG x kLuG
+constexpr int kLuTL = 8;     // tile slots per thread and dimension
+constexpr int kLuNmax = kLuG * kLuTL;  // 256
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const unsigned long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+  return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Power of two s with s * amax in [0.5, 1); 1 for amax == 0 or non-finite.
+__device__ __forceinline__ double pow2_scale(double amax) {
+  if (!(amax > 0.0) || !isfinite(amax)) return 1.0;
+  int e;
+  frexp(amax, &e);
+  return ldexp(1.0, -e);
+}
+
+template <typename T, int TLC_LDS>
+struct LuTile {
+  static constexpr int TLC_REG = kLuTL - TLC_LDS;
+  T r[kLuTL][TLC_REG > 0 ? TLC_REG : 1];
+  T* lds;  // column-major [32*TLC_LDS][256] for columns >= 32*TLC_REG
+  int tr, tc;
+  __device__ __forceinline__ T get(int a, int b) const {
+    if (b < TLC_REG) return r[a][b];
+    return lds[(tc + kLuG * (b - TLC_REG)) * kLuNmax + tr + kLuG * a];
+  }
+  __device__ __forceinline__ void set(int a, int b, T v) {
+    if (b < TLC_REG) r[a][b] = v;
+    else lds[(tc + kLuG * (b - TLC_REG)) * kLuNmax + tr + kLuG * a] = v;
+  }
+};
+
+template <typename T, int TLC_LDS>
+__global__ void __launch_bounds__(1024)
+lu_factor_kernel(int nb, const int* __restrict__ n_arr, const long* __restrict__ a_off,
+                 const long* __restrict__ x_off, const int* __restrict__ order,
+                 const double* __restrict__ A, T* __restrict__ LU, double* __restrict__ rs,
+                 double* __restrict__ cs, int* __restrict__ piv, int* __restrict__ perm,
+                 int* __restrict__ info, int equilibrate) {
+  __shared__ T lds_mat[TLC_LDS > 0 ? kLuG * TLC_LDS * kLuNmax : 1];
+  __shared__ T rowU[kLuNmax], rowK[kLuNmax], colL[kLuNmax];
+  __shared__ unsigned long long amax_bits[kLuNmax];
+  __shared__ double scl_r[kLuNmax], scl_c[kLuNmax];
+  __shared__ int sh_perm[kLuNmax];
+  __shared__ int sh_piv, sh_info;
+
+  LuTile<T, TLC_LDS> t;
+  t.lds = lds_mat;
+  t.tr = threadIdx.x % kLuG;
+  t.tc = threadIdx.x / kLuG;
+  const int tr = t.tr, tc = t.tc;
+
+  for (int w = blockIdx.x; w < nb; w += gridDim.x) {
+    const int b = order ? order[w] : w;
+    const int n = n_arr[b];
+    if (n < 1 || n > kLuNmax) {  // uniform per block: every thread skips
+      if (threadIdx.x == 0) info[b] = -1;
+      continue;
+    }
+    const int mc = (n + kLuG - 1) / kLuG;  // occupied slots per dimension
+    const double* __restrict__ Ab = A + a_off[b];
+    const long xo = x_off[b];
+    __syncthreads();  // LDS reuse across blocks
+
+    // ---- equilibration scales (powers of two, so scaling is exact) -------
+    if (threadIdx.x < kLuNmax) {
+      sh_perm[threadIdx.x] = threadIdx.x;
+      amax_bits[threadIdx.x] = 0ull;
+      scl_r[threadIdx.x] = 1.0;
+      scl_c[threadIdx.x] = 1.0;
+    }
+    if (threadIdx.x == 0) sh_info = 0;
+    __syncthreads();
+    if (equilibrate) {
+      // row maxima
+#pragma unroll
+      for (int a = 0; a < kLuTL; ++a) {
+        const int i = tr + kLuG * a;
+        if (a >= mc || i >= n) continue;
+        double m = 0.0;
+#pragma unroll
+        for (int bb = 0; bb < kLuTL; ++bb) {
+          const int j = tc + kLuG * bb;
+          if (bb < mc && j < n) m = fmax(m, fabs(Ab[i + (long)j * n]));
+        }
+        atomicMax(&amax_bits[i], (unsigned long long)__double_as_longlong(m));
+      }
+      __syncthreads();
+      if (threadIdx.x < n) {
+        scl_r[threadIdx.x] = pow2_scale(__longlong_as_double((long long)amax_bits[threadIdx.x]));
+        amax_bits[threadIdx.x] = 0ull;
+      }
+      __syncthreads();
+      // column maxima of the row-scaled matrix
+#pragma unroll
+      for (int bb = 0; bb < kLuTL; ++bb) {
+        const int j = tc + kLuG * bb;
+        if (bb >= mc || j >= n) continue;
+        double m = 0.0;
+#pragma unroll
+        for (int a = 0; a < kLuTL; ++a) {
+          const int i = tr + kLuG * a;
+          if (a < mc && i < n) m = fmax(m, fabs(Ab[i + (long)j * n]) * scl_r[i]);
+        }
+        atomicMax(&amax_bits[j], (unsigned long long)__double_as_longlong(m));
+      }
+      __syncthreads();
+      if (threadIdx.x < n)
+        scl_c[threadIdx.x] = pow2_scale(__longlong_as_double((long long)amax_bits[threadIdx.x]));
+      __syncthreads();
+    }
+
+    // ---- load the (scaled) block into the tile ---------------------------
+#pragma unroll
+    for (int a = 0; a < kLuTL; ++a) {
+#pragma unroll
+      for (int bb = 0; bb < kLuTL; ++bb) {
+        const int i = tr + kLuG * a, j = tc + kLuG * bb;
+        T v = 0;
+        if (a < mc && bb < mc && i < n && j < n) v = (T)(Ab[i + (long)j * n] * scl_r[i] * scl_c[j]);
+        t.set(a, bb, v);
+      }
+    }
+    __syncthreads();
+
+    // ---- right-looking elimination --------------------------------------
+    for (int k = 0; k < n; ++k) {
+      const int kb = k / kLuG, kc = k % kLuG;
+      if (tc == kc) {
+        double best = -1.0;
+        int bi = n;
+#pragma unroll
+        for (int a = 0; a < kLuTL; ++a) {
+          const int i = tr + kLuG * a;
+          T v = 0;
+#pragma unroll
+          for (int bb = 0; bb < kLuTL; ++bb)
+            if (bb == kb) v = t.get(a, bb);
+          const double av = fabs((double)v);
+          if (a < mc && i >= k && i < n && av > best) {
+            best = av;
+            bi = i;
+          }
+        }
+#pragma unroll
+        for (int o = 16; o >= 1; o >>= 1) {
+          const double ob = __shfl_xor(best, o, 32);
+          const int oi = __shfl_xor(bi, o, 32);
+          if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+          }
+        }
+        if (tr == 0) {
+          sh_piv = bi;
+          if (best == 0.0 && sh_info == 0) sh_info = k + 1;
+        }
+      }
+      __syncthreads();
+      const int p = sh_piv;
+      const int ka = k / kLuG, kr = k % kLuG, pa = p / kLuG, pr = p % kLuG;
+      // publish pivot row (old row p), old row k and old column k
+#pragma unroll
+      for (int bb = 0; bb < kLuTL; ++bb) {
+        const int j = tc + kLuG * bb;
+        if (bb < mc && j < n) {
+          if (tr == pr) {
+            T v = 0;
+#pragma unroll
+            for (int a = 0; a < kLuTL; ++a)
+              if (a == pa) v = t.get(a, bb);
+            rowU[j] = v;
+          }
+          if (tr == kr) {
+            T v = 0;
+#pragma unroll
+            for (int a = 0; a < kLuTL; ++a)
+              if (a == ka) v = t.get(a, bb);
+            rowK[j] = v;
+          }
+        }
+      }
+      if (tc == kc) {
+#pragma unroll
+        for (int a = 0; a < kLuTL; ++a) {
+          const int i = tr + kLuG * a;
+          T v = 0;
+#pragma unroll
+          for (int bb = 0; bb < kLuTL; ++bb)
+            if (bb == kb) v = t.get(a, bb);
+          if (a < mc && i < n) colL[i] = v;
+        }
+      }
+      if (threadIdx.x == 0) {
+        piv[xo + k] = p;
+        const int tmp = sh_perm[k];
+        sh_perm[k] = sh_perm[p];
+        sh_perm[p] = tmp;
+      }
+      __syncthreads();
+      const T ukk = rowU[k];
+      const T inv = (ukk != (T)0) ? (T)1 / ukk : (T)0;
+      T u[kLuTL];
+#pragma unroll
+      for (int bb = 0; bb < kLuTL; ++bb) {
+        const int j = tc + kLuG * bb;
+        u[bb] = (bb < mc && j > k && j < n) ? rowU[j] : (T)0;
+      }
+#pragma unroll
+      for (int a = 0; a < kLuTL; ++a) {
+        const int i = tr + kLuG * a;
+        if (a >= mc || i >= n || i < k) continue;
+        if (i == k || i == p) {
+#pragma unroll
+          for (int bb = 0; bb < kLuTL; ++bb) {
+            const int j = tc + kLuG * bb;
+            if (bb < mc && j < n) t.set(a, bb, i == k ? rowU[j] : rowK[j]);
+          }
+        }
+        if (i == k) continue;
+        const T raw = (i == p) ? rowK[k] : colL[i];
+        const T l = raw * inv;
+        if (tc == kc) {
+#pragma unroll
+          for (int bb = 0; bb < kLuTL; ++bb)
+            if (bb == kb) t.set(a, bb, l);
+        }
+#pragma unroll
+        for (int bb = 0; bb < kLuTL; ++bb)
+          if (bb < mc) t.set(a, bb, t.get(a, bb) - l * u[bb]);
+      }
+      // The next step's pivot search reads only registers/LDS tile entries
+      // that this thread itself updated; the next LDS publish happens after
+      // that step's first barrier, which every thread reaches only once it
+      // is done with rowU/rowK/colL here.
+    }
+    __syncthreads();
+
+    // ---- write factors, permutation, scales ----------------------------
+    T* __restrict__ Lb = LU + a_off[b];
+#pragma unroll
+    for (int a = 0; a < kLuTL; ++a) {
+#pragma unroll
+      for (int bb = 0; bb < kLuTL; ++bb) {
+        const int i = tr + kLuG * a, j = tc + kLuG * bb;
+        if (a < mc && bb < mc && i < n && j < n) Lb[i + (long)j * n] = t.get(a, bb);
+      }
+    }
+    if (threadIdx.x < n) {
+      perm[xo + threadIdx.x] = sh_perm[threadIdx.x];
+      if (rs) rs[xo + threadIdx.x] = scl_r[threadIdx.x];
+      if (cs) cs[xo + threadIdx.x] = scl_c[threadIdx.x];
+    }
+    if (threadIdx.x == 0) info[b] = sh_info;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One wavefront per block: lane owns rows lane + 64 m, m < 4.
+constexpr int kSlots = 4;
+
+// In-wave row permutation: w_new[row i] = w_old[row perm[i]], with row r held
+// by lane r % 64 in slot r / 64.
+__device__ __forceinline__ void permute_rows(double (&w)[kSlots], const int (&src_lane)[kSlots],
+                                             const int (&src_slot)[kSlots], int n, int lane) {
+  double out[kSlots];
+#pragma unroll
+  for (int m = 0; m < kSlots; ++m) {
+    double acc = 0.0;
+#pragma unroll
+    for (int ms = 0; ms < kSlots; ++ms) {
+      const double t = __shfl(w[ms], src_lane[m], 64);
+      if (ms == src_slot[m]) acc = t;
+    }
+    out[m] = (64 * m + lane < n) ? acc : 0.0;
+  }
+#pragma unroll
+  for (int m = 0; m < kSlots; ++m) w[m] = out[m];
+}
+
+template <typename T>
+__device__ __forceinline__ void trsv_lu(const T* __restrict__ L, int n, int lane, double (&w)[kSlots]) {
+  const int mc = (n + 63) / 64;
+  // forward: unit lower triangular
+#pragma unroll
+  for (int m = 0; m < kSlots; ++m) {
+    if (m >= mc) break;
+    const int kend = min(64, n - 64 * m);
+#pragma unroll 4
+    for (int kk = 0; kk < kend; ++kk) {
+      const int k = 64 * m + kk;
+      const double wk = readlane_f64(w[m], kk);
+      const T* __restrict__ col = L + (long)k * n;
+#pragma unroll
+      for (int mm = m; mm < kSlots; ++mm) {
+        const int i = 64 * mm + lane;
+        if (mm < mc && i > k && i < n) w[mm] = fma(-(double)col[i], wk, w[mm]);
+      }
+    }
+  }
+  // backward: upper triangular
+#pragma unroll
+  for (int m = kSlots - 1; m >= 0; --m) {
+    if (m >= mc) continue;
+    const int kend = min(64, n - 64 * m);
+    for (int kk = kend - 1; kk >= 0; --kk) {
+      const int k = 64 * m + kk;
+      const T* __restrict__ col = L + (long)k * n;
+      if (lane == kk) w[m] = w[m] / (double)col[k];
+      const double wk = readlane_f64(w[m], kk);
+#pragma unroll
+      for (int mm = 0; mm <= m; ++mm) {
+        const int i = 64 * mm + lane;
+        if (i < k) w[mm] = fma(-(double)col[i], wk, w[mm]);
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+lu_solve_kernel(const int* __restrict__ n_arr, const long* __restrict__ a_off,
+                const long* __restrict__ x_off, const double* __restrict__ A,
+                const T* __restrict__ LU, const double* __restrict__ rs,
+                const double* __restrict__ cs, const int* __restrict__ perm,
+                const double* x, double* y, int refine, double tol, int* __restrict__ iters) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int n = n_arr[b];
+  if (n < 1 || n > 64 * kSlots) {
+    if (iters && lane == 0) iters[b] = -1;
+    return;
+  }
+  const long xo = x_off[b];
+  const T* __restrict__ L = LU + a_off[b];
+  const double* __restrict__ Ab = A ? A + a_off[b] : nullptr;
+  const int* __restrict__ pm = perm + xo;
+
+  double xv[kSlots], yv[kSlots], w[kSlots], csv[kSlots];
+#pragma unroll
+  for (int m = 0; m < kSlots; ++m) {
+    const int i = 64 * m + lane;
+    xv[m] = i < n ? x[xo + i] : 0.0;
+    csv[m] = (cs && i < n) ? cs[xo + i] : 1.0;
+  }
+  // w = P (rs .* x): row i of the factor is row perm[i] of A.
+  int src_lane[kSlots], src_slot[kSlots];
+#pragma unroll
+  for (int m = 0; m < kSlots; ++m) {
+    const int i = 64 * m + lane;
+    const int src = i < n ? pm[i] : 0;
+    src_lane[m] = src & 63;
+    src_slot[m] = src >> 6;
+    const double rsv = (rs && i < n) ? rs[xo + i] : 1.0;
+    w[m] = xv[m] * rsv;  // still in A's row order
+  }
+  permute_rows(w, src_lane, src_slot, n, lane);
+  trsv_lu<T>(L, n, lane, w);
+#pragma unroll
+  for (int m = 0; m < kSlots; ++m) yv[m] = w[m] * csv[m];
+
+  int it = 0;
+  double anorm = -1.0;
+  for (; it < refine; ++it) {
+    // r = x - A y in FP64 (first sweep also measures ||A||_inf)
+    double r[kSlots], rowabs[kSlots];
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m) {
+      r[m] = xv[m];
+      rowabs[m] = 0.0;
+    }
+    const int mc = (n + 63) / 64;
+#pragma unroll
+    for (int m0 = 0; m0 < kSlots; ++m0) {
+      if (m0 >= mc) break;
+      const int kend = min(64, n - 64 * m0);
+      for (int kk = 0; kk < kend; ++kk) {
+        const int j = 64 * m0 + kk;
+        const double yj = readlane_f64(yv[m0], kk);
+        const double* __restrict__ col = Ab + (long)j * n;
+#pragma unroll
+        for (int mm = 0; mm < kSlots; ++mm) {
+          const int i = 64 * mm + lane;
+          if (mm < mc && i < n) {
+            const double aij = col[i];
+            r[mm] = fma(-aij, yj, r[mm]);
+            if (anorm < 0.0) rowabs[mm] += fabs(aij);
+          }
+        }
+      }
+    }
+    double rn = 0.0, yn = 0.0, an = 0.0;
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m) {
+      rn = fmax(rn, fabs(r[m]));
+      yn = fmax(yn, fabs(yv[m]));
+      an = fmax(an, rowabs[m]);
+    }
+    rn = wave_max_f64(rn);
+    yn = wave_max_f64(yn);
+    if (anorm < 0.0) anorm = wave_max_f64(an);
+    if (rn <= tol * anorm * yn) break;
+    // dy = solve(r)
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m) {
+      const int i = 64 * m + lane;
+      w[m] = r[m] * ((rs && i < n) ? rs[xo + i] : 1.0);
+    }
+    permute_rows(w, src_lane, src_slot, n, lane);
+    trsv_lu<T>(L, n, lane, w);
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m) yv[m] += w[m] * csv[m];
+  }
+  if (iters) iters[b] = it;
+#pragma unroll
+  for (int m = 0; m < kSlots; ++m) {
+    const int i = 64 * m + lane;
+    if (i < n) y[xo + i] = yv[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64)
+block_gemv_kernel(const int* __restrict__ m_arr, const int* __restrict__ k_arr,
+                  const long* __restrict__ m_off, const long* __restrict__ x_off,
+                  const long* __restrict__ y_off, const double* __restrict__ M,
+                  const double* __restrict__ x, double* __restrict__ y, double alpha,
+                  double beta) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int m = m_arr[b], kk = k_arr[b];
+  if (m < 0 || m > 64 * kSlots || kk < 0) return;
+  const double* __restrict__ Mb = M + m_off[b];
+  const double* __restrict__ xb = x + x_off[b];
+  double acc[kSlots];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) acc[s] = 0.0;
+  const int mc = (m + 63) / 64;
+  int j = 0;
+  for (; j + 4 <= kk; j += 4) {
+    double xj[4], v[4][kSlots];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xj[u] = xb[j + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int s = 0; s < kSlots; ++s) {
+        const int i = 64 * s + lane;
+        v[u][s] = (s < mc && i < m) ? Mb[i + (long)(j + u) * m] : 0.0;
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int s = 0; s < kSlots; ++s) acc[s] = fma(v[u][s], xj[u], acc[s]);
+  }
+  for (; j < kk; ++j) {
+    const double xj = xb[j];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+      const int i = 64 * s + lane;
+      if (s < mc && i < m) acc[s] = fma(Mb[i + (long)j * m], xj, acc[s]);
+    }
+  }
+  double* __restrict__ yb = y + y_off[b];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    const int i = 64 * s + lane;
+    if (s < mc && i < m) yb[i] = beta == 0.0 ? alpha * acc[s] : fma(alpha, acc[s], beta * yb[i]);
+  }
+}
+
+static int num_cus() {
+  static int cached = 0;
+  if (cached == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+    cached = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  }
+  return cached;
+}
+
+}  // namespace f3d
+
+using namespace f3d;
+
+extern "C" int f3d_lu_factor_batched(int precision, int64_t nb, int32_t nmax, const int32_t* n,
+                                     const int64_t* a_off, const int64_t* x_off,
+                                     const int32_t* order, const double* A, double* lu64,
+                                     float* lu32, double* rs, double* cs, int32_t* piv,
+                                     int32_t* perm, int32_t* info, void* stream) {
+  F3D_REQUIRE(precision == F3D_F64 || precision == F3D_F32C, F3D_EINVAL,
+              "lu_factor: unknown precision %d", precision);
+  F3D_REQUIRE(nb >= 0 && nb < (1ll << 31), F3D_EINVAL, "lu_factor: bad batch count");
+  F3D_REQUIRE(nmax >= 1 && nmax <= kLuNmax, F3D_EINVAL, "lu_factor: nmax=%d outside [1,%d]",
+              nmax, kLuNmax);
+  if (nb == 0) return F3D_OK;
+  F3D_REQUIRE(n && a_off && x_off && A && piv && perm && info, F3D_EINVAL,
+              "lu_factor: NULL argument");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const unsigned grid = (unsigned)(nb < num_cus() ? nb : num_cus());
+  if (precision == F3D_F64) {
+    F3D_REQUIRE(lu64 != nullptr, F3D_EINVAL, "lu_factor: lu64 is NULL");
+    F3D_LAUNCH((lu_factor_kernel<double, 2>), dim3(grid), dim3(1024), 0, st, (int)nb, n,
+               (const long*)a_off, (const long*)x_off, order, A, lu64, (double*)nullptr,
+               (double*)nullptr, piv, perm, info, 0);
+  } else {
+    F3D_REQUIRE(lu32 && rs && cs, F3D_EINVAL, "lu_factor: lu32/rs/cs is NULL");
+    F3D_LAUNCH((lu_factor_kernel<float, 0>), dim3(grid), dim3(1024), 0, st, (int)nb, n,
+               (const long*)a_off, (const long*)x_off, order, A, lu32, rs, cs, piv, perm, info,
+               1);
+  }
+  return F3D_OK;
+}
+
+extern "C" int f3d_lu_solve_batched(int precision, int64_t nb, int32_t nmax, const int32_t* n,
+                                    const int64_t* a_off, const int64_t* x_off, const double* A,
+                                    const double* lu64, const float* lu32, const double* rs,
+                                    const double* cs, const int32_t* perm, const double* x,
+                                    double* y, int32_t refine, double tol, int32_t* iters,
+                                    void* stream) {
+  F3D_REQUIRE(precision == F3D_F64 || precision == F3D_F32C, F3D_EINVAL,
+              "lu_solve: unknown precision %d", precision);
+  F3D_REQUIRE(nb >= 0 && nb < (1ll << 31), F3D_EINVAL, "lu_solve: bad batch count");
+  F3D_REQUIRE(nmax >= 1 && nmax <= 64 * kSlots, F3D_EINVAL, "lu_solve: nmax=%d outside [1,%d]",
+              nmax, 64 * kSlots);
+  F3D_REQUIRE(refine >= 0, F3D_EINVAL, "lu_solve: negative refine");
+  if (nb == 0) return F3D_OK;
+  F3D_REQUIRE(n && a_off && x_off && perm && x && y, F3D_EINVAL, "lu_solve: NULL argument");
+  F3D_REQUIRE(refine == 0 || A != nullptr, F3D_EINVAL, "lu_solve: refinement needs A");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (precision == F3D_F64) {
+    F3D_REQUIRE(lu64 != nullptr, F3D_EINVAL, "lu_solve: lu64 is NULL");
+    F3D_LAUNCH((lu_solve_kernel<double>), dim3((unsigned)nb), dim3(64), 0, st, n,
+               (const long*)a_off, (const long*)x_off, A, lu64, rs, cs, perm, x, y, refine, tol,
+               iters);
+  } else {
+    F3D_REQUIRE(lu32 && rs && cs, F3D_EINVAL, "lu_solve: lu32/rs/cs is NULL");
+    F3D_LAUNCH((lu_solve_kernel<float>), dim3((unsigned)nb), dim3(64), 0, st, n,
+               (const long*)a_off, (const long*)x_off, A, lu32, rs, cs, perm, x, y, refine, tol,
+               iters);
+  }
+  return F3D_OK;
+}
+
+extern "C" int f3d_block_gemv_batched(int64_t nb, int32_t mmax, const int32_t* m,
+                                      const int32_t* k, const int64_t* m_off,
+                                      const int64_t* x_off, const int64_t* y_off,
+                                      const double* M, const double* x, double* y, double alpha,
+                                      double beta, void* stream) {
+  F3D_REQUIRE(nb >= 0 && nb < (1ll << 31), F3D_EINVAL, "block_gemv: bad batch count");
+  F3D_REQUIRE(mmax >= 1 && mmax <= 64 * kSlots, F3D_EINVAL,
+              "block_gemv: mmax=%d outside [1,%d]", mmax, 64 * kSlots);
+  if (nb == 0) return F3D_OK;
+  F3D_REQUIRE(m && k && m_off && x_off && y_off && M && x && y, F3D_EINVAL,
+              "block_gemv: NULL argument");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  F3D_LAUNCH(block_gemv_kernel, dim3((unsigned)nb), dim3(64), 0, st, m, k, (const long*)m_off,
+             (const long*)x_off, (const long*)y_off, M, x, y, alpha, beta);
+  return F3D_OK;
+}

@@ -1,0 +1,182 @@
+/*
+ * f3d.h -- C ABI of libf3d.so, the MI355X (gfx950) implementation of the
+ * fiber3d GMRES-matvec hot path.
+ *
+ * Conventions (all entry points):
+ *   - Every pointer argument is a DEVICE pointer owned by the caller, except
+ *     where a comment says "host".
+ *   - A 3-vector field over n points is addressed as p[i*sp + c*sc] for point
+ *     i in [0,n) and component c in {0,1,2}:  AoS "xyzxyz" is (sp,sc)=(3,1),
+ *     SoA "xx..yy..zz.." is (sp,sc)=(1,n).  All arrays of one side (sources,
+ *     targets, output) share that side's (sp,sc).
+ *   - Work is enqueued asynchronously on `stream` (a hipStream_t; NULL = the
+ *     default stream).  No call synchronises the device.
+ *   - Return 0 on success, a negative F3D_E* code on failure; the message is
+ *     then available from f3d_last_error() (thread-local).
+ *   - Scratch memory comes from the caller's `ws` buffer; the required size is
+ *     given by the matching *_workspace_size() query.
+ */
+#ifndef F3D_H
+#define F3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define F3D_OK 0
+#define F3D_EINVAL (-1)    /* invalid argument (size, pointer, enum)      */
+#define F3D_EHIP (-2)      /* a HIP runtime call failed                    */
+#define F3D_EWORKSPACE (-3)/* workspace missing or too small               */
+
+/* ---- pair-sum kernels ------------------------------------------------ */
+#define F3D_OSEEN 0     /* Stokeslet: u_i = sum_j (1/(8 pi eta)) [f/r + d(d.f)/r^3] */
+#define F3D_ROTLET 1    /* rotlet:    u_i = sum_j (1/(8 pi eta)) (tau x d)/r^3      */
+#define F3D_STRESSLET 2 /* double layer: u_i = sum_j (-3/(4 pi eta)) (d.q)(d.n) d/r^5 */
+
+#define F3D_F64 0  /* FP64 pair evaluation and accumulation                    */
+#define F3D_F32C 1 /* FP32 pair evaluation, compensated (Kahan) per 64-source
+                      tile, tile partials combined in FP64; output FP64        */
+
+/* flags for f3d_pairsum */
+#define F3D_ACCUMULATE 1u     /* out += result instead of out = result         */
+#define F3D_CLAMP_NEAR 2u     /* F32C only: instead of the reference's
+                                 regularisation r -> sqrt(r^2+reg^2) for
+                                 r < eps, clamp 1/r at 1/eps (one instruction
+                                 less per pair; identical results whenever no
+                                 non-excluded pair is closer than eps)        */
+
+/* Returns a short description of the last error on the calling thread. */
+const char* f3d_last_error(void);
+/* ABI version (major*100 + minor). */
+int f3d_abi_version(void);
+
+/* Bytes of workspace f3d_pairsum needs for these sizes. */
+size_t f3d_pairsum_workspace_size(int kernel, int precision, int64_t ns, int64_t nt);
+
+/*
+ * Source-target pair sum, matrix free.  Replaces
+ *   kernels.oseen_kernel_source_target_numba      (python/kernels/kernels.py:794-846)
+ *   kernels.rotlet_kernel_source_target_numba     (python/kernels/kernels.py:856-908)
+ *   kernels.stresslet_kernel_source_target_numba  (python/kernels/kernels.py:1186-1217)
+ *   kernels.stresslet_kernel_times_normal_times_density_numba
+ *                                                 (python/kernels/kernels.py:1123-1183;
+ *      call with the same points as sources and targets: the i==j term is
+ *      exactly zero for the stresslet, as in the reference's j != i loop)
+ * and, with src_w != NULL, the weight multiply of tstep_utils.flow_fibers
+ *   (python/tstep/tstep_utils.py:653-700).
+ *
+ *   d = t_i - s_j, r = |d|; if r < eps then r <- sqrt(r^2 + reg^2) and d is
+ *   kept (numba semantics, kernels.py:830-831).
+ *
+ *   src_pos   [ns] positions of the sources
+ *   src_nrm   [ns] normals (STRESSLET only, else NULL)
+ *   src_den   [ns] densities f / tau / q
+ *   src_w     [ns] optional scalar weight per source (density is multiplied)
+ *   trg_pos   [nt] target positions
+ *   excl_lo/excl_hi [nt] optional int32 half-open source-index range
+ *             [lo_i, hi_i) whose pairs are EXCLUDED for target i (same-fiber
+ *             exclusion, or lo=i,hi=i+1 to drop the self term); both NULL or
+ *             both set.  Replaces the add-then-subtract of the per-fiber self
+ *             block (tstep_utils.py:703-729, :812-851).
+ *   out       [nt] result, scaled by the kernel constant
+ */
+int f3d_pairsum(int kernel, int precision, uint32_t flags,
+                int64_t ns, const double* src_pos, const double* src_nrm,
+                const double* src_den, const double* src_w, int64_t s_sp, int64_t s_sc,
+                int64_t nt, const double* trg_pos, int64_t t_sp, int64_t t_sc,
+                const int32_t* excl_lo, const int32_t* excl_hi,
+                double eta, double reg, double eps,
+                double* out, int64_t o_sp, int64_t o_sc,
+                void* ws, size_t ws_bytes, void* stream);
+
+/* ---- periphery (confining shell) operator ---------------------------- */
+
+size_t f3d_periphery_workspace_size(int precision, int64_t n);
+
+/*
+ * Matrix-free periphery double-layer operator, replacing the dense
+ * shell_stresslet + shell_complementary matvecs of
+ *   tstep.A_body_fiber_hyd (python/tstep/tstep.py:1097-1104), whose matrices
+ *   are built at python/tstep/tstep.py:328-345 and
+ *   periphery.Periphery.get_singularity_subtraction_vectors
+ *   (python/periphery/periphery.py:75-98):
+ *
+ *   out_i = sum_{j != i} S(x_i - x_j; n_j) q_j
+ *           - sum_c (q_{i,c} / w_i) e_{c,i} - q_i / w_i + n_i sum_j n_j . q_j
+ *           [+ add_i]
+ *
+ *   pos, nrm, q, ex, ey, ez, add, out: [n] 3-vectors with the same (sp,sc);
+ *   w: [n] quadrature weights; add may be NULL.
+ */
+int f3d_periphery_apply(int precision, int64_t n, const double* pos, const double* nrm,
+                        const double* w, const double* ex, const double* ey,
+                        const double* ez, const double* q, const double* add,
+                        int64_t sp, int64_t sc, double eta, double reg, double eps,
+                        double* out, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- batched dense LU over blocks of uneven size --------------------- */
+/*
+ * A batch of `nb` square blocks; block b has order n[b] <= 256 and is stored
+ * column-major at A + a_off[b] (leading dimension n[b]).  n, a_off, x_off and
+ * order are DEVICE arrays (int32, int64, int64, int32).  `order` (nullable)
+ * lists the blocks in the sequence the persistent workgroups pick them up
+ * (largest first balances uneven sizes); NULL means 0..nb-1.
+ *
+ * f3d_lu_factor_batched replaces tstep_utils.build_block_diagonal_lu_
+ * preconditioner_fiber (python/tstep/tstep_utils.py:1405-1419), i.e.
+ * scipy.linalg.lu_factor (LAPACK dgetrf, partial pivoting, first maximal
+ * |pivot| wins ties) per fiber, in ONE persistent launch.
+ *   precision F3D_F64: A is factored in FP64 into lu64 (offsets a_off).
+ *   precision F3D_F32C: A is first equilibrated by powers of two,
+ *     As = diag(rs) A diag(cs), then factored in FP32 into lu32 (offsets
+ *     a_off); rs, cs are written at offsets x_off.  Solves refine in FP64.
+ *   piv  [x_off]: LAPACK-style 0-based row interchanges (row k <-> piv[k]).
+ *   perm [x_off]: the resulting row permutation, row k of LU is row perm[k]
+ *                 of A (used by the solves).
+ *   info [nb]: 0, or k+1 for the first exactly-zero pivot U(k,k).
+ */
+int f3d_lu_factor_batched(int precision, int64_t nb, int32_t nmax, const int32_t* n,
+                          const int64_t* a_off, const int64_t* x_off, const int32_t* order,
+                          const double* A, double* lu64, float* lu32, double* rs, double* cs,
+                          int32_t* piv, int32_t* perm, int32_t* info, void* stream);
+
+/*
+ * Solve A_b y_b = x_b for every block (x, y at offsets x_off; y may alias x).
+ * Replaces the per-fiber scipy.linalg.lu_solve loop of tstep.P_inv
+ * (python/tstep/tstep.py:985-988) with one launch, one wavefront per block.
+ *   F3D_F64: one forward/backward substitution with lu64.
+ *   F3D_F32C: substitution with the FP32 factors (arithmetic in FP64), then up
+ *     to `refine` sweeps of FP64 iterative refinement, r = x - A y with the
+ *     FP64 matrix A, stopping per block once
+ *     ||r||_inf <= tol * ||A||_inf * ||y||_inf.
+ *   iters [nb] (nullable): refinement sweeps used per block.
+ */
+int f3d_lu_solve_batched(int precision, int64_t nb, int32_t nmax, const int32_t* n,
+                         const int64_t* a_off, const int64_t* x_off, const double* A,
+                         const double* lu64, const float* lu32, const double* rs,
+                         const double* cs, const int32_t* perm, const double* x, double* y,
+                         int32_t refine, double tol, int32_t* iters, void* stream);
+
+/*
+ * y_b = alpha * M_b x_b + beta * y_b for a batch of dense COLUMN-major blocks
+ * of shape m[b] x k[b] (leading dimension m[b]) stored at M + m_off[b]; x at
+ * x_off[b], y at y_off[b] (y must not alias x).  m[b] <= mmax <= 1024.
+ * Replaces the block-diagonal CSR apply As_fibers.dot(XT)
+ * (python/tstep/tstep.py:1173; python/multi_fibers/multi_fiber.py:47-78) and
+ * the per-fiber force operator / tension projection products
+ * (python/tstep/tstep.py:1036, :1160, :1171).
+ */
+int f3d_block_gemv_batched(int64_t nb, int32_t mmax, const int32_t* m, const int32_t* k,
+                           const int64_t* m_off, const int64_t* x_off, const int64_t* y_off,
+                           const double* M, const double* x, double* y, double alpha,
+                           double beta, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* F3D_H */

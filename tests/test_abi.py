@@ -1,0 +1,65 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry point
+include/f3d.h declares, and rejects bad arguments before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+from fiber3d_amd import _lib as L
+
+HEADER = os.path.join(ROOT, "include", "f3d.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(f3d_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_binding_table():
+    assert _declared() == sorted(L.SIGNATURES)
+
+
+def test_library_exports_all_symbols():
+    lib = L.load()
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert lib.f3d_abi_version() == 100
+
+
+def test_bad_arguments_rejected_on_host():
+    lib = L.load()
+    null = None
+    rc = lib.f3d_pairsum(7, L.F64, 0, 10, null, null, null, null, 3, 1, 10, null, 3, 1, null,
+                         null, 1.0, 5e-3, 1e-5, null, 3, 1, null, 0, null)
+    assert rc == L.EINVAL and b"unknown kernel" in lib.f3d_last_error()
+    rc = lib.f3d_pairsum(L.OSEEN, 9, 0, 10, null, null, null, null, 3, 1, 10, null, 3, 1, null,
+                         null, 1.0, 5e-3, 1e-5, null, 3, 1, null, 0, null)
+    assert rc == L.EINVAL and b"precision" in lib.f3d_last_error()
+    rc = lib.f3d_pairsum(L.OSEEN, L.F64, 0, 10, null, null, null, null, 3, 1, 10, null, 3, 1,
+                         null, null, -1.0, 5e-3, 1e-5, null, 3, 1, null, 0, null)
+    assert rc == L.EINVAL and b"eta" in lib.f3d_last_error()
+    # lo given without hi
+    rc = lib.f3d_pairsum(L.OSEEN, L.F64, 0, 10, null, null, null, null, 3, 1, 10, null, 3, 1,
+                         ctypes.c_void_p(8), null, 1.0, 5e-3, 1e-5, null, 3, 1, null, 0, null)
+    assert rc == L.EINVAL
+    rc = lib.f3d_lu_factor_batched(L.F64, 4, 300, null, null, null, null, null, null, null,
+                                   null, null, null, null, null, null)
+    assert rc == L.EINVAL and b"nmax" in lib.f3d_last_error()
+    rc = lib.f3d_lu_solve_batched(L.F32C, 4, 64, null, null, null, null, null, null, null,
+                                  null, null, null, null, -1, 1e-12, null, null)
+    assert rc == L.EINVAL
+    rc = lib.f3d_block_gemv_batched(3, 0, null, null, null, null, null, null, null, null, 1.0,
+                                    0.0, null)
+    assert rc == L.EINVAL
+    assert lib.f3d_pairsum_workspace_size(0, 5, 10, 10) == 0
+    assert lib.f3d_pairsum_workspace_size(0, L.F32C, 65536, 65536) > 65536 * 32
+
+
+def test_no_cpu_fallback_without_library(monkeypatch, tmp_path):
+    monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(L, "_lib", None)
+    with pytest.raises(L.F3DError):
+        L.load()

@@ -1,0 +1,147 @@
+"""Pins of the FP64 oracle's pair sums against facts the mathematics fixes.
+
+None of these retypes the oracle's formula: each compares it with a physical
+identity or an independent form, chosen so that a wrong constant, a dropped
+term, a wrong sign or a transposed operand fails.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel2
+from fiber3d_amd.geometry import shape_gallery_sphere
+from oracle import kernels as K
+from oracle import periphery as P
+
+
+def _sphere(N, R):
+    x, n = shape_gallery_sphere(N, R)
+    return x, n, np.full(N, 4 * np.pi * R * R / N)
+
+
+def test_oseen_stokes_drag_identity():
+    # Mean of the Stokeslet over a sphere of radius R about the source is
+    # F / (6 pi eta R): pins 1/(8 pi eta) and the 1 : 1 weight of the two terms
+    # (dropping d(d.f)/r^3 would give F/(8 pi eta R)).
+    R, eta = 2.0, 1.3
+    x, _, _ = _sphere(4000, R)
+    f = np.array([[0.3, -1.2, 0.7]])
+    u = K.oseen_kernel_source_target(np.zeros((1, 3)), x, f, eta=eta).reshape(-1, 3)
+    assert rel2(u.mean(0), f[0] / (6 * np.pi * eta * R)) < 2e-4
+
+
+def _fd_jacobian(fun, t, h=1e-4):
+    J = np.zeros((3, 3))
+    for c in range(3):
+        e = np.zeros(3)
+        e[c] = h
+        J[:, c] = (fun(t + e) - fun(t - e)) / (2 * h)
+    return J
+
+
+def test_oseen_divergence_free_and_reciprocal():
+    rng = np.random.default_rng(10)
+    s = rng.normal(size=(7, 3))
+    f = rng.normal(size=(7, 3))
+    t0 = np.array([2.5, -1.0, 0.4])
+    J = _fd_jacobian(lambda t: K.oseen_kernel_source_target(s, t[None], f), t0)
+    u = K.oseen_kernel_source_target(s, t0[None], f)
+    assert abs(np.trace(J)) < 1e-7 * np.linalg.norm(J)
+    # reciprocity of the symmetric tensor: g . G(s->t) f == f . G(t->s) g
+    t = rng.normal(size=(5, 3)) + 4.0
+    g = rng.normal(size=(5, 3))
+    a = np.dot(g.ravel(), K.oseen_kernel_source_target(s, t, f))
+    b = np.dot(f.ravel(), K.oseen_kernel_source_target(t, s, g))
+    assert abs(a - b) < 1e-12 * (abs(a) + abs(b))
+    assert np.all(np.isfinite(u))
+
+
+def test_oseen_regularised_coincident_pair():
+    # kernels.py:830-831: r < eps -> sqrt(r^2 + reg^2), d kept: a coincident
+    # pair gives f / (8 pi eta reg) exactly, a pair at r just above eps is raw.
+    f = np.array([[1.0, -2.0, 0.5]])
+    u = K.oseen_kernel_source_target(np.zeros((1, 3)), np.zeros((1, 3)), f, eta=2.0, reg=5e-3)
+    np.testing.assert_allclose(u, f[0] / (8 * np.pi * 2.0 * 5e-3), rtol=1e-14)
+
+
+def test_rotlet_is_half_curl_of_oseen():
+    rng = np.random.default_rng(11)
+    s = rng.normal(size=(6, 3))
+    tau = rng.normal(size=(6, 3))
+    t0 = np.array([1.7, 2.2, -0.9])
+    J = _fd_jacobian(lambda t: K.oseen_kernel_source_target(s, t[None], tau, eta=0.7), t0)
+    curl = np.array([J[2, 1] - J[1, 2], J[0, 2] - J[2, 0], J[1, 0] - J[0, 1]])
+    r = K.rotlet_kernel_source_target(s, t0[None], tau, eta=0.7)
+    assert rel2(r, 0.5 * curl) < 1e-7
+
+
+def test_stresslet_gauss_identity():
+    # Double layer of a constant density over a closed surface with outward
+    # normals (d = x - y): +e inside, 0 outside, +e/2 on the surface.
+    R = 2.0
+    x, n, w = _sphere(4000, R)
+    e = np.array([0.2, -0.5, 1.0])
+    q = np.tile(e, (x.shape[0], 1)) * w[:, None]
+    inside = K.stresslet_kernel_source_target(x, np.array([[0.1, 0.2, -0.3]]), n, q)
+    outside = K.stresslet_kernel_source_target(x, np.array([[3.0, 0.5, 0.1]]), n, q)
+    assert rel2(inside, e) < 1e-3
+    assert np.linalg.norm(outside) < 1e-3 * np.linalg.norm(e)
+    on = K.stresslet_kernel_times_normal_times_density(x, n, q).reshape(-1, 3)
+    assert rel2(on.mean(0), 0.5 * e) < 3e-2
+
+
+def test_stresslet_divergence_free():
+    rng = np.random.default_rng(12)
+    s = rng.normal(size=(5, 3))
+    nrm = rng.normal(size=(5, 3))
+    q = rng.normal(size=(5, 3))
+    t0 = np.array([2.0, 1.5, -2.5])
+    J = _fd_jacobian(lambda t: K.stresslet_kernel_source_target(s, t[None], nrm, q), t0)
+    assert abs(np.trace(J)) < 1e-7 * np.linalg.norm(J)
+
+
+def test_self_sum_equals_dense_matrix():
+    rng = np.random.default_rng(13)
+    x = rng.normal(size=(60, 3))
+    nrm = rng.normal(size=(60, 3))
+    q = rng.normal(size=(60, 3))
+    dense = K.stresslet_kernel_times_normal(x, nrm) @ q.ravel()
+    assert rel2(K.stresslet_kernel_times_normal_times_density(x, nrm, q), dense) < 1e-13
+
+
+def test_complementary_rank_one():
+    rng = np.random.default_rng(14)
+    x = rng.normal(size=(9, 3))
+    nrm = rng.normal(size=(9, 3))
+    q = rng.normal(size=(9, 3))
+    Nk = K.complementary_kernel(x, nrm)
+    assert np.linalg.matrix_rank(Nk) == 1
+    assert rel2(K.complementary_kernel_times_density(x, nrm, q), Nk @ q.ravel()) < 1e-14
+
+
+def test_oseen_tensor_dense_matches_sum():
+    # kernels.py:31-116 (dense, r <= eps regularised incl. the diagonal) against
+    # the pair sum with the same points (coincident i == j regularised).
+    rng = np.random.default_rng(15)
+    x = rng.normal(size=(40, 3))
+    f = rng.normal(size=(40, 3))
+    G = K.oseen_tensor(x)
+    assert rel2(G @ f.ravel(), K.oseen_kernel_source_target(x, x, f)) < 1e-13
+    np.testing.assert_allclose(np.diag(G)[:3], 1.0 / (8 * np.pi * 5e-3), rtol=1e-14)
+
+
+def test_shell_dense_equals_matrix_free_and_invariant():
+    R = 3.0
+    x, nout, w = _sphere(300, R)
+    n = -nout  # the shell's normals point inward (tstep.py:293)
+    ex, ey, ez = P.singularity_subtraction_vectors(x, n, w)
+    rng = np.random.default_rng(16)
+    q = rng.normal(size=(300, 3))
+    M = P.shell_matrix(x, n, w, ex, ey, ez)
+    assert rel2(P.shell_apply(x, n, w, ex, ey, ez, q), M @ q.ravel()) < 1e-13
+    # singularity subtraction: M (w * e_c) = -e_c + n sum_j n_j . (w_j e_c)
+    for c in range(3):
+        e = np.zeros((300, 3))
+        e[:, c] = 1.0
+        got = P.shell_apply(x, n, w, ex, ey, ez, e * w[:, None]).reshape(-1, 3)
+        expect = -e + n * np.sum(n * e * w[:, None])
+        assert np.max(np.abs(got - expect)) < 1e-12
