@@ -44,13 +44,18 @@ SIGNATURES = {
     "f3d_periphery_apply": (ctypes.c_int, [
         ctypes.c_int, _i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
         _i64, _i64, _f64, _f64, _f64, _c_p, _c_p, _sz, _c_p]),
+    "f3d_stresslet_matrix": (ctypes.c_int, [
+        _i64, _c_p, _c_p, _i64, _i64, _f64, _f64, _f64, _c_p, _i64, _c_p]),
     "f3d_lu_factor_batched": (ctypes.c_int, [
         ctypes.c_int, _i64, _i32, _c_p, _c_p, _c_p, _c_p,
-        _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+        _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "f3d_lu_solve_batched": (ctypes.c_int, [
-        ctypes.c_int, _i64, _i32, _c_p, _c_p, _c_p, _c_p,
-        _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
+        ctypes.c_int, _i64, _i32, _c_p, _c_p, _c_p, _c_p,   # n, a_off, x_off, A
+        _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,           # lu64 lu32 rs cs perm x y
         _i32, _f64, _c_p, _c_p]),
+    "f3d_gemm_bf16": (ctypes.c_int, [
+        ctypes.c_int, _i64, _i64, _i64, _i32, _c_p, _i64, _i64, _c_p, _i64, _i64,
+        _c_p, _i64, _i64, _c_p, _c_p, _c_p, ctypes.c_float, _c_p]),
     "f3d_block_gemv_batched": (ctypes.c_int, [
         _i64, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _f64, _f64, _c_p]),
 }

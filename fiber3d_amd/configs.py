@@ -1,0 +1,118 @@
+"""Seeded synthetic configurations of the benchmark workloads.
+
+Fibers: orientations uniform on the sphere (normalised Gaussian), centres
+uniform in a ball (c = R U^(1/3) h/|h|), rejected when any node lies within
+`min_sep` of a node of an already placed fiber (node-node separation).  Nodes
+are c + (s_m - L/2) t with s_m = (1 + alpha_m) L/2 on the discretisation's
+parameter grid.  The periphery is the Fibonacci sphere of
+shape_gallery.py:7-32 with inward normals (tstep.py:293) and equal quadrature
+weights 4 pi R^2 / N.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+from .geometry import shape_gallery_sphere
+
+
+@dataclass
+class FiberSpec:
+    num_fibers: int
+    num_points: object          # int, or a list of allowed node counts
+    length: float
+    ball_radius: float
+    E: float = 0.01
+    epsilon: float = 1e-3
+    min_sep: float = 0.2
+    end_force: float = 1.0       # plus-end force along the end tangent
+
+
+@dataclass
+class Config:
+    name: str
+    seed: int
+    fibers: FiberSpec
+    periphery_nodes: int = 0
+    periphery_radius: float = 0.0
+    eta: float = 1.0
+    dt: float = 1e-3
+    penalty: float = 500.0
+    nfd: int = 4
+    discretisation: str = "reference"   # uniform nodes + FD ('reference') or 'chebyshev'
+    gmres_tol: float = 1e-8
+    restart: int = 150
+    maxiter: int = 200
+    steps: int = 1
+    extra: dict = field(default_factory=dict)
+
+
+def config(i, discretisation="reference"):
+    """The five workloads (0: small FP64 plain, 2: batched LU, 3: time series, 4: large)."""
+    if i == 0:
+        return Config("small", 0, FiberSpec(32, 32, 1.0, 4.0), 1024, 5.0,
+                      discretisation=discretisation)
+    if i == 2:
+        return Config("batched_lu", 2, FiberSpec(2048, [16, 24, 32, 48, 64], 1.0, 20.0),
+                      discretisation=discretisation)
+    if i == 3:
+        return Config("time_series", 3, FiberSpec(1000, 32, 1.0, 8.0), 8000, 10.0, steps=400,
+                      discretisation=discretisation)
+    if i == 4:
+        return Config("large", 4, FiberSpec(4096, 64, 1.5, 14.0), 32768, 16.0, steps=50,
+                      discretisation=discretisation)
+    raise ValueError(f"no fiber-system config {i}")
+
+
+def _alpha(N, discretisation):
+    if discretisation == "reference":
+        return np.linspace(-1.0, 1.0, N)
+    return -np.cos(np.pi * np.arange(N) / (N - 1))
+
+
+def make_fibers(spec: FiberSpec, seed: int, discretisation="reference"):
+    """List of (N_k, 3) node arrays placed without node-node overlap."""
+    rng = np.random.default_rng(seed)
+    placed = []
+    pts = np.zeros((0, 3))
+    tree = None
+    tree_n = 0
+    pending = []
+    attempts = 0
+    while len(placed) < spec.num_fibers:
+        attempts += 1
+        if attempts > 200 * spec.num_fibers:
+            raise RuntimeError("fiber placement failed: packing too dense")
+        if isinstance(spec.num_points, (list, tuple)):
+            N = int(rng.choice(spec.num_points))
+        else:
+            N = int(spec.num_points)
+        g = rng.normal(size=3)
+        t = g / np.linalg.norm(g)
+        h = rng.normal(size=3)
+        c = spec.ball_radius * rng.uniform() ** (1.0 / 3.0) * h / np.linalg.norm(h)
+        s = (1.0 + _alpha(N, discretisation)) * spec.length / 2.0
+        x = c[None, :] + (s - spec.length / 2.0)[:, None] * t[None, :]
+        ok = True
+        if tree is not None and tree.query(x, k=1, distance_upper_bound=spec.min_sep)[0].min() < spec.min_sep:
+            ok = False
+        if ok and pending:
+            P = np.concatenate(pending)
+            d = np.linalg.norm(x[:, None, :] - P[None, :, :], axis=2)
+            ok = d.min() >= spec.min_sep
+        if not ok:
+            continue
+        placed.append(x)
+        pending.append(x)
+        if sum(p.shape[0] for p in pending) > 4096:
+            pts = np.concatenate([pts] + pending)
+            pending = []
+            tree = cKDTree(pts)
+    return placed
+
+
+def make_periphery(n, radius):
+    """Shell nodes, inward normals, equal weights."""
+    x, nout = shape_gallery_sphere(n, radius)
+    w = np.full(n, 4.0 * np.pi * radius * radius / n)
+    return x, -nout, w

@@ -492,3 +492,52 @@ extern "C" int f3d_periphery_apply(int precision, int64_t n, const double* pos,
                       sc, nullptr, nullptr, eta, reg, eps, out, sp, sc, ws, ws_bytes, &epi,
                       stream);
 }
+
+// ---------------------------------------------------------------------------
+// Dense stresslet . normal matrix (python/kernels/kernels.py:1063-1120):
+// block (i, j != i) = (-3/(4 pi eta)) (d . n_j) / r^5 d d^T, d = x_i - x_j,
+// zero diagonal blocks; row-major 3n x 3n with leading dimension ld.
+namespace f3d {
+__global__ void __launch_bounds__(256)
+stresslet_matrix_kernel(int n, const double* __restrict__ pos, const double* __restrict__ nrm,
+                        long sp, long sc, double factor, double reg2, double eps2,
+                        double* __restrict__ M, long ld) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= n) return;
+  const long bi = (long)i * sp, bj = (long)j * sp;
+  const double dx = pos[bi] - pos[bj];
+  const double dy = pos[bi + sc] - pos[bj + sc];
+  const double dz = pos[bi + 2 * sc] - pos[bj + 2 * sc];
+  double c = 0.0;
+  if (i != j) {
+    double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < eps2) r2 += reg2;
+    const double r = sqrt(r2);
+    const double dn = dx * nrm[bj] + dy * nrm[bj + sc] + dz * nrm[bj + 2 * sc];
+    c = factor * dn / (r2 * r2 * r);
+  }
+  const double d[3] = {dx, dy, dz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double* row = M + (3L * i + a) * ld + 3L * j;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) row[b] = c * d[a] * d[b];
+  }
+}
+}  // namespace f3d
+
+extern "C" int f3d_stresslet_matrix(int64_t n, const double* pos, const double* nrm, int64_t sp,
+                                    int64_t sc, double eta, double reg, double eps, double* M,
+                                    int64_t ld, void* stream) {
+  F3D_REQUIRE(n >= 0 && n < (1ll << 30), F3D_EINVAL, "stresslet_matrix: bad n");
+  F3D_REQUIRE(ld >= 3 * n, F3D_EINVAL, "stresslet_matrix: ld < 3n");
+  F3D_REQUIRE(eta > 0.0, F3D_EINVAL, "stresslet_matrix: eta must be positive");
+  if (n == 0) return F3D_OK;
+  F3D_REQUIRE(pos && nrm && M, F3D_EINVAL, "stresslet_matrix: NULL argument");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  F3D_LAUNCH(stresslet_matrix_kernel, dim3((unsigned)ceil_div(n, 256), (unsigned)n), dim3(256), 0,
+             st, (int)n, pos, nrm, (long)sp, (long)sc, -3.0 / (4.0 * kPi * eta), reg * reg,
+             eps * eps, M, (long)ld);
+  return F3D_OK;
+}

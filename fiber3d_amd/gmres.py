@@ -1,0 +1,114 @@
+"""Right-preconditioned restarted GMRES in FP64 on the GPU.
+
+Semantics of bpm_utilities/gmres.py:94-108 over SciPy's gmres as called at
+python/tstep/tstep.py:1209-1216: solve (A P^-1) y = b, return x = P^-1 y,
+stop when the residual norm <= tol ||b|| (atol = 0), restart length and
+maximum restart cycles as given.  An initial guess x0 is taken in x-space
+(r0 = b - A x0), so a predicted solution can seed the solve; a guess that is
+not finite, or whose residual is larger than ||b||, is dropped and the solve
+starts from zero (the plain method stays the fallback).
+
+Arnoldi uses classical Gram-Schmidt with one re-orthogonalisation (CGS2) as
+two matrix-vector products with the basis, all on the device in FP64.  The
+(restart+1) x restart Hessenberg least-squares problem is updated with Givens
+rotations on the host (j+2 scalars per iteration).
+
+An optional `inner` operator (e.g. the FP32-compensated matvec) is used for
+the Arnoldi products while the true residual that decides convergence and
+restarts is always recomputed with the FP64 `matvec`.
+"""
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+
+@dataclass
+class GmresResult:
+    x: torch.Tensor
+    info: int
+    iterations: int
+    restarts: int
+    residual: float          # true ||b - A x|| / ||b|| in FP64
+    guess_used: bool
+    history: list
+
+
+def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, inner=None,
+          callback=None):
+    Pinv = precond if precond is not None else (lambda v, out=None: v.clone())
+    op = inner if inner is not None else matvec
+    n = b.numel()
+    dev = b.device
+    bnorm = float(torch.linalg.norm(b))
+    hist = []
+    if bnorm == 0.0:
+        return GmresResult(torch.zeros_like(b), 0, 0, 0, 0.0, False, hist)
+    guess_used = False
+    x = torch.zeros_like(b)
+    r = b.clone()
+    if x0 is not None:
+        if bool(torch.isfinite(x0).all()):
+            r0 = b - matvec(x0)
+            if float(torch.linalg.norm(r0)) < bnorm:
+                x = x0.clone()
+                r = r0
+                guess_used = True
+    m = min(restart, n)
+    V = torch.empty((m + 1, n), dtype=torch.float64, device=dev)
+    total = 0
+    restarts = 0
+    w = torch.empty_like(b)
+    z = torch.empty_like(b)
+    for _cycle in range(maxiter):
+        beta = float(torch.linalg.norm(r))
+        hist.append(beta / bnorm)
+        if beta <= tol * bnorm:
+            return GmresResult(x, 0, total, restarts, beta / bnorm, guess_used, hist)
+        V[0] = r / beta
+        H = np.zeros((m + 1, m))
+        cs = np.zeros(m)
+        sn = np.zeros(m)
+        g = np.zeros(m + 1)
+        g[0] = beta
+        j_done = 0
+        for j in range(m):
+            Pinv(V[j], out=z)
+            op(z, out=w)
+            # CGS2 against V[0..j]
+            Vj = V[:j + 1]
+            h1 = torch.mv(Vj, w)
+            w -= torch.mv(Vj.t(), h1)
+            h2 = torch.mv(Vj, w)
+            w -= torch.mv(Vj.t(), h2)
+            hn = torch.linalg.norm(w)
+            hcol = torch.cat((h1 + h2, hn.reshape(1))).cpu().numpy()
+            H[:j + 2, j] = hcol
+            if hcol[j + 1] > 0:
+                V[j + 1] = w / hcol[j + 1]
+            for i in range(j):
+                t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                H[i, j] = t
+            den = math.hypot(H[j, j], H[j + 1, j])
+            cs[j], sn[j] = (H[j, j] / den, H[j + 1, j] / den) if den > 0 else (1.0, 0.0)
+            H[j, j] = den
+            H[j + 1, j] = 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            total += 1
+            j_done = j + 1
+            est = abs(g[j + 1]) / bnorm
+            hist.append(est)
+            if callback is not None:
+                callback(est)
+            if est <= tol or hcol[j + 1] == 0.0:
+                break
+        yk = np.linalg.solve(np.triu(H[:j_done, :j_done]), g[:j_done]) if j_done else np.zeros(0)
+        upd = torch.mv(V[:j_done].t(), torch.from_numpy(yk).to(dev))
+        x += Pinv(upd)
+        r = b - matvec(x)
+        restarts += 1
+    rn = float(torch.linalg.norm(r)) / bnorm
+    return GmresResult(x, 0 if rn <= tol else total, total, restarts, rn, guess_used, hist)

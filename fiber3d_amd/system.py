@@ -1,0 +1,329 @@
+"""GPU-resident implicit hydrodynamic step of free fibers in a confining shell.
+
+Mirrors tstep.time_step_hydro (python/tstep/tstep.py:628-1216) for fibers with
+force/torque-free ends inside a spherical periphery:
+  * per-step assembly, batched over fibers of equal node count on the GPU
+    (fiber.form_linear_operator / compute_RHS / apply_BC_rectangular /
+    force_operator, python/fiber/fiber.py:322-961, :1399-1431);
+  * the matvec A_body_fiber_hyd (tstep.py:1003-1178) from libf3d kernels:
+    batched block GEMVs for the force operator, the tension projection and
+    A_sq XT, the Oseen / stresslet pair sums with same-fiber exclusion
+    (replacing the add-then-subtract of the dense self blocks,
+    tstep_utils.py:703-729, :812-851) and the matrix-free periphery operator;
+  * the block preconditioner P_inv (tstep.py:950-990): batched fiber LU solves
+    and a dense shell LU;
+  * right-preconditioned FP64 GMRES (gmres.py in this package).
+Unknown vector layout is the reference's: [shell density (3 Nper, AoS) |
+per fiber X(N), Y(N), Z(N), T(N)].
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import fiber_matrices as FMX
+from .dense import BatchedLU, BlockGemv, BlockSet
+from .kernels import _stream, device, pairsum, periphery_apply
+
+
+def _scatter_blocks(buf, offs_h, blocks):
+    """Write equal-length rows blocks[b] at buf[offs_h[b]:...]; a single strided
+    copy when the offsets are evenly spaced, else one copy per block."""
+    B, n = blocks.shape
+    offs_h = np.asarray(offs_h, np.int64)
+    if B == 1 or np.all(np.diff(offs_h) == n):
+        buf[offs_h[0]:offs_h[0] + B * n].view(B, n).copy_(blocks)
+    else:
+        for b in range(B):
+            buf[offs_h[b]:offs_h[b] + n].copy_(blocks[b])
+
+
+def _c0c1(epsilon, eta):
+    return -np.log(np.e * epsilon**2) / (8.0 * np.pi * eta), 2.0 / (8.0 * np.pi * eta)
+
+
+class Shell:
+    """Static periphery: nodes, inward normals, weights, subtraction vectors,
+    dense preconditioner (built and LU-factored once, as tstep.py:278-356)."""
+
+    def __init__(self, x, normals, weights, eta=1.0, reg=5e-3, eps=1e-5, dense_pc=True):
+        dev = device()
+        self.n = x.shape[0]
+        self.pos = torch.from_numpy(np.ascontiguousarray(x, np.float64)).to(dev)
+        self.nrm = torch.from_numpy(np.ascontiguousarray(normals, np.float64)).to(dev)
+        self.w = torch.from_numpy(np.ascontiguousarray(weights, np.float64)).to(dev)
+        self.eta, self.reg, self.eps = eta, reg, eps
+        # e_c = stresslet self-sum of w * e_c (periphery.py:75-98)
+        self.e = []
+        for c in range(3):
+            dens = torch.zeros_like(self.pos)
+            dens[:, c] = self.w
+            self.e.append(pairsum("stresslet", self.pos, dens, self.pos, src_nrm=self.nrm,
+                                  eta=eta, reg=reg, eps=eps))
+        self.lu = None
+        if dense_pc:
+            self.build_dense_pc()
+
+    def dense_matrix(self):
+        """M of tstep.py:328-345 on the GPU (3N x 3N, AoS ordering)."""
+        N = self.n
+        M = torch.empty((3 * N, 3 * N), dtype=torch.float64, device=self.pos.device)
+        rc = L.load().f3d_stresslet_matrix(N, L.ptr(self.pos), L.ptr(self.nrm), 3, 1,
+                                           float(self.eta), float(self.reg), float(self.eps),
+                                           L.ptr(M), 3 * N, _stream())
+        L.check(rc, "f3d_stresslet_matrix")
+        Mv = M.view(N, 3, N, 3)
+        idx = torch.arange(N, device=M.device)
+        winv = 1.0 / self.w
+        for c in range(3):
+            # - e_c(i) / w_i in column (i, c); - 1/w_i on the diagonal
+            Mv[idx, :, idx, c] -= self.e[c] * winv[:, None]
+            Mv[idx, c, idx, c] -= winv
+        nflat = self.nrm.reshape(-1)
+        M += torch.outer(nflat, nflat)
+        return M
+
+    def build_dense_pc(self):
+        M = self.dense_matrix()
+        self.lu, self.piv = torch.linalg.lu_factor(M)
+        del M
+
+    def apply(self, q, add=None, out=None, precision="f64"):
+        return periphery_apply(self.pos, self.nrm, self.w, self.e[0], self.e[1], self.e[2], q,
+                               add=add, eta=self.eta, reg=self.reg, eps=self.eps,
+                               precision=precision, out=out)
+
+    def precond(self, r, out):
+        out.copy_(torch.linalg.lu_solve(self.lu, self.piv, r.reshape(-1, 1)).reshape(-1))
+        return out
+
+
+class FiberSystem:
+    def __init__(self, fibers, shell=None, eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
+                 penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
+                 lengths=None, pair_precision="f64", lu_precision="f64", lu_refine=6,
+                 lu_tol=1e-13, reg=5e-3, eps=1e-5):
+        self.dev = device()
+        self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
+        self.c0, self.c1 = _c0c1(epsilon, eta)
+        self.nfd, self.kind = nfd, discretisation
+        self.reg, self.eps = reg, eps
+        self.end_force = end_force
+        self.pair_precision = pair_precision
+        self.lu_precision, self.lu_refine, self.lu_tol = lu_precision, lu_refine, lu_tol
+        sizes = np.array([np.asarray(x).shape[0] for x in fibers], dtype=np.int64)
+        if sizes.min() < 5:
+            raise ValueError("fibers need at least 5 nodes")
+        self.sizes = sizes
+        self.nfib = len(sizes)
+        self.offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        self.npts = int(self.offs[-1])
+        self.shell = shell
+        self.nper = 0 if shell is None else shell.n
+        self.size = 3 * self.nper + 4 * self.npts
+        self.lengths_h = np.full(self.nfib, 1.0) if lengths is None else np.asarray(lengths, float)
+        self.lengths = torch.from_numpy(self.lengths_h).to(self.dev)
+        # node positions (npts, 3), AoS
+        self.X = torch.from_numpy(np.ascontiguousarray(np.concatenate(fibers), np.float64)).to(self.dev)
+        # buckets of equal node count
+        self.buckets = {}
+        for N in sorted(set(sizes.tolist())):
+            ids = np.flatnonzero(sizes == N)
+            fm = FMX.get(int(N), nfd, discretisation)
+            t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(self.dev)
+            self.buckets[int(N)] = {
+                "ids": ids, "ids_t": torch.from_numpy(ids).to(self.dev),
+                "D0": [t(D) for D in fm.D0], "P": t(fm.P_cheb), "w0": t(fm.weights0),
+                "alpha": t(fm.alpha),
+                # node index of each bucket fiber's first node
+                "start": torch.from_numpy(self.offs[ids]).to(self.dev),
+            }
+        # same-fiber exclusion ranges for fiber targets
+        lo = np.repeat(self.offs[:-1], sizes).astype(np.int32)
+        hi = np.repeat(self.offs[1:], sizes).astype(np.int32)
+        self.excl = (torch.from_numpy(lo).to(self.dev), torch.from_numpy(hi).to(self.dev))
+        # block layouts
+        self.bs = BlockSet(4 * sizes)  # A_sq blocks (4N x 4N)
+        fo = self.offs[:-1]
+        # force operator: 3 row blocks per fiber (N x 4N) -> SoA f (3, npts)
+        mF = np.repeat(sizes, 3)
+        kF = 4 * mF
+        xF = np.repeat(4 * fo, 3)
+        yF = (np.arange(3)[None, :] * self.npts + fo[:, None]).ravel()
+        self.gF = BlockGemv(mF, kF, xF, yF)
+        # tension projection: per component c, (4N-14) x N blocks
+        self.gQ = [BlockGemv(4 * sizes - 14, sizes, c * self.npts + fo, 4 * fo) for c in range(3)]
+        self.gA = BlockGemv(4 * sizes, 4 * sizes, 4 * fo, 4 * fo)
+        # work buffers
+        z = lambda n: torch.zeros(n, dtype=torch.float64, device=self.dev)
+        self.f_soa = z(3 * self.npts)
+        self.v_soa = z(3 * self.npts)
+        self.v_shell = torch.zeros((max(self.nper, 1), 3), dtype=torch.float64, device=self.dev)
+        self.prepare()
+
+    # ------------------------------------------------------------------
+    def _soa(self, buf):
+        """(npts, 3) view of an SoA (3, npts) buffer."""
+        return buf.view(3, self.npts).t()
+
+    def prepare(self):
+        """Per-step assembly: derivatives, A_sq, RHS, force operator, tension
+        projection, explicit flows, LU factors (tstep.py:689-941)."""
+        dev = self.dev
+        E, dt, c0, c1, p = self.E, self.dt, self.c0, self.c1, self.penalty
+        A_buf = torch.empty(self.bs.nelem, dtype=torch.float64, device=dev)
+        F_buf = torch.empty(self.gF.nelem, dtype=torch.float64, device=dev)
+        Q_buf = [torch.empty(g.nelem, dtype=torch.float64, device=dev) for g in self.gQ]
+        self.weights = torch.empty(self.npts, dtype=torch.float64, device=dev)
+        self.xs = torch.empty((self.npts, 3), dtype=torch.float64, device=dev)
+        self._rhs_parts = {}
+        self._ctx = {}
+        for N, bk in self.buckets.items():
+            ids_t = bk["ids_t"]
+            B = ids_t.numel()
+            node = bk["start"][:, None] + torch.arange(N, device=dev)[None, :]  # (B, N)
+            X = self.X[node]  # (B, N, 3)
+            sc = (2.0 / self.lengths[ids_t])[:, None, None]
+            Ds = [bk["D0"][k][None] * sc ** (k + 1) for k in range(4)]  # (B, N, N)
+            xs, xss, xsss = (torch.matmul(Ds[k], X) for k in range(3))
+            I = torch.eye(N, dtype=torch.float64, device=dev)[None]
+            D1, D2, D3, D4 = Ds
+            dl = lambda v, D: v[:, :, None] * D
+            diag = lambda v: torch.diag_embed(v)
+            A = torch.zeros((B, 4 * N, 4 * N), dtype=torch.float64, device=dev)
+            for a in range(3):
+                for b in range(a, 3):
+                    if a == b:
+                        blk = I / dt + E * c0 * dl(1.0 + xs[..., a] ** 2, D4) \
+                            + E * c1 * dl(1.0 - xs[..., a] ** 2, D4)
+                    else:
+                        blk = E * (c0 - c1) * dl(xs[..., a] * xs[..., b], D4)
+                    A[:, a * N:(a + 1) * N, b * N:(b + 1) * N] = blk
+                    A[:, b * N:(b + 1) * N, a * N:(a + 1) * N] = blk
+                A[:, a * N:(a + 1) * N, 3 * N:] = -c0 * (2.0 * dl(xs[..., a], D1) + diag(xss[..., a])) \
+                    - c1 * diag(xss[..., a])
+                A[:, 3 * N:, a * N:(a + 1) * N] = -(c1 + 7.0 * c0) * E * dl(xss[..., a], D4) \
+                    - 6.0 * c0 * E * dl(xsss[..., a], D3) - p * dl(xs[..., a], D1)
+            A[:, 3 * N:, 3 * N:] = -2.0 * c0 * D2 + (c1 + c0) * diag((xss**2).sum(-1))
+            # boundary rows (force/torque-free ends, fiber.py:836-959)
+            Brow = torch.zeros((B, 14, 4 * N), dtype=torch.float64, device=dev)
+            for a in range(3):
+                Brow[:, a, a * N:(a + 1) * N] = E * D3[:, 0]
+                Brow[:, a, 3 * N] = -xs[:, 0, a]
+                Brow[:, 3, a * N:(a + 1) * N] = -E * D2[:, 0] * xss[:, 0, a, None]
+                Brow[:, 4 + a, a * N:(a + 1) * N] = D2[:, 0]
+                Brow[:, 7 + a, a * N:(a + 1) * N] = -E * D3[:, -1]
+                Brow[:, 7 + a, 4 * N - 1] = xs[:, -1, a]
+                Brow[:, 10, a * N:(a + 1) * N] = E * D2[:, -1] * xss[:, -1, a, None]
+                Brow[:, 11 + a, a * N:(a + 1) * N] = D2[:, -1]
+            Brow[:, 3, 3 * N] = -1.0
+            Brow[:, 10, 4 * N - 1] = 1.0
+            A_sq = torch.cat((torch.matmul(bk["P"], A), Brow), dim=1)  # (B, 4N, 4N)
+            # column-major blocks into the global buffer at their fiber offsets
+            _scatter_blocks(A_buf, self.bs.a_off_h[bk["ids"]], A_sq.transpose(1, 2).reshape(B, -1))
+            # force operator rows c: [-E D4 in block c | diag(xss_c) + diag(xs_c) D1]
+            Fb = torch.zeros((B, 3, N, 4 * N), dtype=torch.float64, device=dev)
+            for a in range(3):
+                Fb[:, a, :, a * N:(a + 1) * N] = -E * D4
+                Fb[:, a, :, 3 * N:] = diag(xss[..., a]) + dl(xs[..., a], D1)
+            f_off = self.gF.m_off_h.reshape(self.nfib, 3)[bk["ids"]]
+            # the three row blocks of a fiber are adjacent: one (B, 3*N*4N) copy
+            _scatter_blocks(F_buf, f_off[:, 0], Fb.transpose(2, 3).reshape(B, -1))
+            # tension projection Q_c = P_cheb [e_c ; diag(xs_c) D1] (4N-14 x N)
+            P = bk["P"]
+            for a in range(3):
+                Tc = torch.zeros((B, 4 * N, N), dtype=torch.float64, device=dev)
+                Tc[:, a * N:(a + 1) * N] = I
+                Tc[:, 3 * N:] = dl(xs[..., a], D1)
+                Qc = torch.matmul(P, Tc)  # (B, 4N-14, N)
+                _scatter_blocks(Q_buf[a], self.gQ[a].m_off_h[bk["ids"]],
+                                Qc.transpose(1, 2).reshape(B, -1))
+            w = bk["w0"][None, :] * (self.lengths[ids_t][:, None] / 2.0)
+            self.weights[node.reshape(-1)] = w.reshape(-1)
+            self.xs[node.reshape(-1)] = xs.reshape(-1, 3)
+            self._ctx[N] = (node, X, xs, xss, D1, P, B)
+        self.A_buf, self.F_buf, self.Q_buf = A_buf, F_buf, Q_buf
+        self._build_rhs()
+        self.lu = BatchedLU(self.bs, self.A_buf, precision=self.lu_precision,
+                            refine=self.lu_refine, tol=self.lu_tol)
+
+    def _build_rhs(self):
+        """RHS: explicit flows of the plus-end forces and per-fiber RHS
+        (tstep.py:794-846, tstep_utils.py:1238-1336, fiber.py:473-582)."""
+        dev = self.dev
+        # explicit force: plus end along the end tangent
+        force = torch.zeros((self.npts, 3), dtype=torch.float64, device=dev)
+        last = torch.from_numpy(self.offs[1:] - 1).to(dev)
+        force[last] = self.end_force * self.xs[last]
+        self.force = force
+        pp = self.pair_precision_rhs = "f64"
+        vff = pairsum("oseen", self.X, force, self.X, src_w=self.weights, excl=self.excl,
+                      eta=self.eta, reg=self.reg, eps=self.eps, precision=pp)
+        rhs = torch.zeros(self.size, dtype=torch.float64, device=dev)
+        if self.shell is not None:
+            vfs = pairsum("oseen", self.X, force, self.shell.pos, src_w=self.weights,
+                          eta=self.eta, reg=self.reg, eps=self.eps, precision=pp)
+            rhs[:3 * self.nper] = -vfs.reshape(-1)
+        dt, c0, c1, p = self.dt, self.c0, self.c1, self.penalty
+        for N, (node, X, xs, xss, D1, P, B) in self._ctx.items():
+            bk = self.buckets[N]
+            fk = force[node]  # (B, N, 3)
+            vk = vff[node]
+            alpha = bk["alpha"]
+            R = torch.zeros((B, 4 * N), dtype=torch.float64, device=dev)
+            xf = (xs * fk).sum(-1)
+            for a in range(3):
+                R[:, a * N:(a + 1) * N] = X[..., a] / dt + vk[..., a] \
+                    + c0 * (fk[..., a] + xs[..., a] * xf) + c1 * (fk[..., a] - xs[..., a] * xf)
+            R[:, 3 * N:] = -p + (xs * torch.matmul(D1, vk)).sum(-1) \
+                + 2.0 * c0 * (xs * torch.matmul(D1, fk)).sum(-1) + (c0 - c1) * (xss * fk).sum(-1)
+            Rsq = torch.zeros((B, 4 * N), dtype=torch.float64, device=dev)
+            Rsq[:, :4 * N - 14] = torch.matmul(P, R[..., None])[..., 0]
+            Fend = fk[:, -1]
+            Rsq[:, 4 * N - 7:4 * N - 4] = Fend
+            Rsq[:, 4 * N - 4] = (Fend * xs[:, -1]).sum(-1)
+            _scatter_blocks(rhs, 3 * self.nper + 4 * self.offs[bk["ids"]], Rsq)
+        self.rhs = rhs
+
+    # ------------------------------------------------------------------
+    def matvec(self, x, out=None, precision=None):
+        """A_body_fiber_hyd (tstep.py:1003-1178) for fibers + shell."""
+        prec = precision or self.pair_precision
+        if out is None:
+            out = torch.empty_like(x)
+        nper3 = 3 * self.nper
+        XT = x[nper3:]
+        # fiber forces f = F XT into SoA (3, npts)
+        self.gF(self.F_buf, XT, self.f_soa)
+        fsoa = self._soa(self.f_soa)
+        vsoa = self._soa(self.v_soa)
+        pairsum("oseen", self.X, fsoa, self.X, src_w=self.weights, excl=self.excl, eta=self.eta,
+                reg=self.reg, eps=self.eps, precision=prec, out=vsoa)
+        if self.shell is not None:
+            q = x[:nper3].view(self.nper, 3)
+            pairsum("oseen", self.X, fsoa, self.shell.pos, src_w=self.weights, eta=self.eta,
+                    reg=self.reg, eps=self.eps, precision=prec, out=self.v_shell)
+            self.shell.apply(q, add=self.v_shell, out=out[:nper3].view(self.nper, 3),
+                             precision=prec)
+            pairsum("stresslet", self.shell.pos, q, self.X, src_nrm=self.shell.nrm, eta=self.eta,
+                    reg=self.reg, eps=self.eps, precision=prec, out=vsoa, accumulate=True)
+        res = out[nper3:]
+        self.gA(self.A_buf, XT, res)
+        for c in range(3):
+            self.gQ[c](self.Q_buf[c], self.v_soa, res, alpha=-1.0, beta=1.0)
+        return out
+
+    def precond(self, x, out=None):
+        """P_inv (tstep.py:950-990)."""
+        if out is None:
+            out = torch.empty_like(x)
+        nper3 = 3 * self.nper
+        if self.shell is not None:
+            self.shell.precond(x[:nper3], out[:nper3])
+        self.lu.solve(x[nper3:], out=out[nper3:])
+        return out
+
+    def dense_blocks_host(self):
+        """Host copies of the per-fiber A_sq blocks (for checks)."""
+        return self.bs.unpack(self.A_buf)

@@ -118,6 +118,17 @@ int f3d_periphery_apply(int precision, int64_t n, const double* pos, const doubl
                         int64_t sp, int64_t sc, double eta, double reg, double eps,
                         double* out, void* ws, size_t ws_bytes, void* stream);
 
+/*
+ * Dense 3n x 3n stresslet-times-normal matrix, row-major with leading
+ * dimension ld: block (i, j != i) = (-3/(4 pi eta)) (d . n_j)/r^5 d d^T with
+ * d = x_i - x_j (regularised for r < eps), zero diagonal blocks.  Replaces
+ * kernels.stresslet_kernel_times_normal_numba (python/kernels/kernels.py:1063-1120),
+ * used to build the dense shell preconditioner (python/tstep/tstep.py:329).
+ */
+int f3d_stresslet_matrix(int64_t n, const double* pos, const double* nrm, int64_t sp,
+                         int64_t sc, double eta, double reg, double eps, double* M, int64_t ld,
+                         void* stream);
+
 /* ---- batched dense LU over blocks of uneven size --------------------- */
 /*
  * A batch of `nb` square blocks; block b has order n[b] <= 256 and is stored
@@ -174,6 +185,21 @@ int f3d_block_gemv_batched(int64_t nb, int32_t mmax, const int32_t* m, const int
                            const int64_t* m_off, const int64_t* x_off, const int64_t* y_off,
                            const double* M, const double* x, double* y, double alpha,
                            double beta, void* stream);
+
+/* ---- BF16 MFMA GEMM for the learned initial-guess network ------------ */
+/*
+ * No counterpart in the reference (the learned GMRES initial guess is new).
+ * C = epi(alpha * A B) with A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]
+ * (FP32 in memory, rounded to BF16 for v_mfma_f32_32x32x16_bf16, FP32
+ * accumulation), C(m,n) = C[m*scm + n*scn].  epi:
+ *   0 store; 1 + bias[n], store pre-activation to Z (nullable), ReLU;
+ *   2 multiply by ReLU'(Zin) (Zin > 0); 3 atomic add (C pre-zeroed; the only
+ *   mode allowed with ksplit > 1); 4 + bias[n].
+ */
+int f3d_gemm_bf16(int epi, int64_t M, int64_t N, int64_t K, int32_t ksplit, const float* A,
+                  int64_t sam, int64_t sak, const float* B, int64_t sbk, int64_t sbn, float* C,
+                  int64_t scm, int64_t scn, const float* bias, float* Z, const float* Zin,
+                  float alpha, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,262 @@
+"""Oracle: one implicit hydrodynamic time step of fibers in a periphery (FP64).
+
+Restates, for free fibers (force/torque-free ends, no bodies) inside a
+spherical shell, the path of tstep.time_step_hydro (python/tstep/tstep.py:628-1216):
+  * explicit flows of the external fiber forces and the system RHS
+    (tstep.py:757-846, tstep_utils.get_fibers_and_bodies_matrices
+    tstep_utils.py:1238-1336, flow_fibers :653-700, self_flow_fibers :812-851);
+  * the matvec A_body_fiber_hyd (tstep.py:1003-1178);
+  * the block preconditioner P_inv (tstep.py:950-990);
+  * right-preconditioned restarted GMRES (bpm_utilities/gmres.py:94-108 over
+    SciPy's gmres, tstep.py:1209-1216), here restated as textbook GMRES
+    (Arnoldi with modified Gram-Schmidt, Givens rotations).
+The unknown vector is [shell density (3 Nper, AoS) | per fiber X, Y, Z, T].
+
+Pinned by tests/test_oracle_tstep.py: the matvec against an independently
+assembled dense operator (config-0-sized), the preconditioner blocks against
+P^-1 (A x) = x, GMRES against a dense direct solve.  The reference's matvec
+and SciPy's GMRES cannot be imported here (stkfmm / mpi4py / numba.errors
+imports), so beyond those invariants the system assembly is parity unpinned.
+"""
+import numpy as np
+import scipy.linalg as sla
+
+from . import fiber as OF
+from . import kernels as K
+from . import periphery as P
+
+
+class HydroSystem:
+    def __init__(self, fibers, shell=None, eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
+                 penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
+                 lengths=None):
+        """fibers: list of (N_k,3) arrays; shell: (x, normals, weights) or None."""
+        self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
+        self.c0, self.c1 = OF.constants(epsilon, eta)
+        self.fibers = [np.asarray(x, float) for x in fibers]
+        self.sizes = np.array([x.shape[0] for x in self.fibers])
+        self.offs = np.concatenate([[0], np.cumsum(self.sizes)])
+        self.lengths = np.full(len(fibers), 1.0) if lengths is None else np.asarray(lengths, float)
+        self.fm = {N: OF.FiberMatrices(N, nfd, discretisation) for N in set(self.sizes.tolist())}
+        self.npts = int(self.offs[-1])
+        self.shell = shell
+        self.nper = 0 if shell is None else shell[0].shape[0]
+        self.size = 3 * self.nper + 4 * self.npts
+        self.trg_fib = np.concatenate(self.fibers)
+        self._setup(end_force)
+
+    # -------------------------------------------------------------------
+    def _setup(self, end_force):
+        eta = self.eta
+        # per-fiber derivatives and matrices (tstep.py:689-718)
+        self.derivs = []
+        self.weights = []
+        self.F_blocks = []
+        self.D1 = []
+        for k, x in enumerate(self.fibers):
+            fm = self.fm[x.shape[0]]
+            L = self.lengths[k]
+            self.derivs.append(OF.derivatives(fm, x, L))
+            self.weights.append(fm.weights(L))
+            self.F_blocks.append(OF.force_operator(fm, x, L, self.E))
+            self.D1.append(fm.Ds(L)[0])
+        # explicit force: plus end (last node) along the end tangent
+        force = np.zeros((self.npts, 3))
+        for k in range(len(self.fibers)):
+            xs = self.derivs[k][0]
+            force[self.offs[k + 1] - 1] = end_force * xs[-1]
+        self.force = force
+        # explicit flows (tstep.py:794-846): fibers -> fibers (self block
+        # removed) and fibers -> shell
+        fw = force * np.concatenate(self.weights)[:, None]
+        vfib2fib = self._fib2fib(fw)
+        rhs = np.zeros(self.size)
+        if self.shell is not None:
+            xs_, ns_, ws_ = self.shell
+            vfib2shell = K.oseen_kernel_source_target(self.trg_fib, xs_, fw, eta)
+            rhs[:3 * self.nper] = -vfib2shell
+            self.ex, self.ey, self.ez = P.singularity_subtraction_vectors(xs_, ns_, ws_, eta)
+        # per-fiber square systems (get_fibers_and_bodies_matrices)
+        self.A_sq = []
+        self.Q = []
+        for k, x in enumerate(self.fibers):
+            N = x.shape[0]
+            fm = self.fm[N]
+            L = self.lengths[k]
+            f_k = force[self.offs[k]:self.offs[k + 1]]
+            v_k = vfib2fib[self.offs[k]:self.offs[k + 1]]
+            A = OF.form_linear_operator(fm, x, L, self.E, self.dt, self.c0, self.c1, self.penalty)
+            R = OF.compute_RHS(fm, x, L, self.dt, self.c0, self.c1, self.penalty, force=f_k,
+                               flow=v_k)
+            A_sq, b = OF.apply_BC_rectangular(fm, x, L, self.E, A, R, end_force=f_k[-1])
+            self.A_sq.append(A_sq)
+            o = 3 * self.nper + 4 * self.offs[k]
+            rhs[o:o + 4 * N] = b
+            # v_on_fib -> P_cheb [v; xs . D1 v], padded with 14 zero rows
+            # (tstep.py:1150-1171)
+            xs = self.derivs[k][0]
+            T = np.zeros((4 * N, 3 * N))
+            for c in range(3):
+                T[c * N:(c + 1) * N, c * N:(c + 1) * N] = np.eye(N)
+                T[3 * N:, c * N:(c + 1) * N] = xs[:, c][:, None] * self.D1[k]
+            Q = np.zeros((4 * N, 3 * N))
+            Q[:4 * N - 14] = fm.P_cheb @ T
+            self.Q.append(Q)
+        self.rhs = rhs
+        # preconditioner factors (tstep.py:936-941, :348-351)
+        self.lu_fib = [sla.lu_factor(A, check_finite=False) for A in self.A_sq]
+        if self.shell is not None:
+            xs_, ns_, ws_ = self.shell
+            self.M_shell = P.shell_matrix(xs_, ns_, ws_, self.ex, self.ey, self.ez, eta)
+            self.lu_shell = sla.lu_factor(self.M_shell)
+
+    def _fib2fib(self, fw):
+        """Oseen flow of fiber forces on fiber nodes minus each fiber's self block
+        (flow_fibers + self_flow_fibers, tstep_utils.py:653-729, :812-851)."""
+        v = K.oseen_kernel_source_target(self.trg_fib, self.trg_fib, fw, self.eta).reshape(-1, 3)
+        for k, x in enumerate(self.fibers):
+            o0, o1 = self.offs[k], self.offs[k + 1]
+            G = K.oseen_tensor(x, self.eta)
+            v[o0:o1] -= (G @ fw[o0:o1].ravel()).reshape(-1, 3)
+        return v
+
+    # -------------------------------------------------------------------
+    def split(self, x):
+        q = x[:3 * self.nper].reshape(-1, 3)
+        XT = x[3 * self.nper:]
+        return q, XT
+
+    def matvec(self, x):
+        """A_body_fiber_hyd for fibers + shell (tstep.py:1003-1178)."""
+        q, XT = self.split(x)
+        res = np.zeros_like(x)
+        f = np.zeros((self.npts, 3))
+        for k in range(len(self.fibers)):
+            N = self.sizes[k]
+            o = 4 * self.offs[k]
+            fk = self.F_blocks[k] @ XT[o:o + 4 * N]
+            f[self.offs[k]:self.offs[k + 1]] = fk.reshape(3, N).T
+        fw = f * np.concatenate(self.weights)[:, None]
+        v_fib = self._fib2fib(fw)
+        if self.shell is not None:
+            xs_, ns_, ws_ = self.shell
+            vfib2shell = K.oseen_kernel_source_target(self.trg_fib, xs_, fw, self.eta)
+            res[:3 * self.nper] = self.M_shell @ q.ravel() + vfib2shell
+            v_fib = v_fib + K.stresslet_kernel_source_target(xs_, self.trg_fib, ns_, q,
+                                                             self.eta).reshape(-1, 3)
+        for k in range(len(self.fibers)):
+            N = self.sizes[k]
+            o = 4 * self.offs[k]
+            vk = v_fib[self.offs[k]:self.offs[k + 1]].T.ravel()
+            res[3 * self.nper + o:3 * self.nper + o + 4 * N] = \
+                self.A_sq[k] @ XT[o:o + 4 * N] - self.Q[k] @ vk
+        return res
+
+    def precond(self, x):
+        """P_inv (tstep.py:950-990): shell LU solve + per-fiber LU solves."""
+        y = np.empty_like(x)
+        if self.shell is not None:
+            y[:3 * self.nper] = sla.lu_solve(self.lu_shell, x[:3 * self.nper])
+        for k in range(len(self.fibers)):
+            N = self.sizes[k]
+            o = 3 * self.nper + 4 * self.offs[k]
+            y[o:o + 4 * N] = sla.lu_solve(self.lu_fib[k], x[o:o + 4 * N])
+        return y
+
+    def dense_operator(self):
+        """The full system matrix assembled from its blocks (independent of matvec)."""
+        n = self.size
+        A = np.zeros((n, n))
+        W = np.concatenate(self.weights)
+        npts, nper = self.npts, self.nper
+        # fiber force operator as one matrix: XT -> fw (AoS, npts x 3)
+        Fbig = np.zeros((3 * npts, 4 * npts))
+        for k in range(len(self.fibers)):
+            N = self.sizes[k]
+            for c in range(3):
+                rows = 3 * (self.offs[k] + np.arange(N)) + c
+                Fbig[rows, 4 * self.offs[k]:4 * self.offs[k] + 4 * N] = \
+                    self.F_blocks[k][c * N:(c + 1) * N] * self.weights[k][:, None]
+        G = K.oseen_tensor(self.trg_fib, self.eta)
+        for k in range(len(self.fibers)):
+            s = slice(3 * self.offs[k], 3 * self.offs[k + 1])
+            G[s, s] = 0.0
+        Vff = G @ Fbig  # fiber velocities (AoS) from XT
+        Qbig = np.zeros((4 * npts, 3 * npts))
+        for k in range(len(self.fibers)):
+            N = self.sizes[k]
+            for c in range(3):
+                cols = 3 * (self.offs[k] + np.arange(N)) + c
+                Qbig[4 * self.offs[k]:4 * self.offs[k] + 4 * N, cols] = self.Q[k][:, c * N:(c + 1) * N]
+        Ablk = sla.block_diag(*self.A_sq)
+        A[3 * nper:, 3 * nper:] = Ablk - Qbig @ Vff
+        if self.shell is not None:
+            xs_, ns_, ws_ = self.shell
+            # fiber -> shell Oseen, shell -> fiber stresslet
+            d = xs_[:, None, :] - self.trg_fib[None, :, :]
+            r = np.linalg.norm(d, axis=2)
+            Gs = (np.eye(3)[None, None] / r[:, :, None, None]
+                  + d[:, :, :, None] * d[:, :, None, :] / r[:, :, None, None] ** 3) / (8 * np.pi * self.eta)
+            Gs = Gs.transpose(0, 2, 1, 3).reshape(3 * nper, 3 * npts)
+            A[:3 * nper, 3 * nper:] = Gs @ Fbig
+            A[:3 * nper, :3 * nper] = self.M_shell
+            d2 = self.trg_fib[:, None, :] - xs_[None, :, :]
+            r2 = np.linalg.norm(d2, axis=2)
+            c = (-3.0 / (4 * np.pi * self.eta)) * np.einsum("ijk,jk->ij", d2, ns_) / r2**5
+            S = c[:, :, None, None] * d2[:, :, :, None] * d2[:, :, None, :]
+            S = S.transpose(0, 2, 1, 3).reshape(3 * npts, 3 * nper)
+            A[3 * nper:, :3 * nper] = -Qbig @ S
+        return A
+
+
+def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None):
+    """Right-preconditioned restarted GMRES: solve (A P^-1) y = b, x = P^-1 y.
+
+    Stops when ||b - A x|| <= tol ||b|| (atol = 0), as bpm_utilities/gmres.py
+    configures SciPy's gmres (tstep.py:1209-1216).  Returns (x, info, iters)."""
+    n = b.size
+    Pinv = precond if precond is not None else (lambda v: v)
+    bnorm = np.linalg.norm(b)
+    if bnorm == 0.0:
+        return np.zeros(n), 0, 0
+    x = np.zeros(n) if x0 is None else np.array(x0, float)
+    total = 0
+    for _outer in range(maxiter):
+        r = b - matvec(x)
+        beta = np.linalg.norm(r)
+        if beta <= tol * bnorm:
+            return x, 0, total
+        V = np.zeros((restart + 1, n))
+        H = np.zeros((restart + 1, restart))
+        cs = np.zeros(restart)
+        sn = np.zeros(restart)
+        g = np.zeros(restart + 1)
+        g[0] = beta
+        V[0] = r / beta
+        j_done = 0
+        for j in range(restart):
+            w = matvec(Pinv(V[j]))
+            for i in range(j + 1):
+                H[i, j] = np.dot(V[i], w)
+                w -= H[i, j] * V[i]
+            H[j + 1, j] = np.linalg.norm(w)
+            if H[j + 1, j] > 0:
+                V[j + 1] = w / H[j + 1, j]
+            for i in range(j):
+                t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                H[i, j] = t
+            den = np.hypot(H[j, j], H[j + 1, j])
+            cs[j], sn[j] = H[j, j] / den, H[j + 1, j] / den
+            H[j, j] = den
+            H[j + 1, j] = 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            total += 1
+            j_done = j + 1
+            if abs(g[j + 1]) <= tol * bnorm:
+                break
+        yk = sla.solve_triangular(H[:j_done, :j_done], g[:j_done])
+        x = x + Pinv(V[:j_done].T @ yk)
+    r = b - matvec(x)
+    return x, (0 if np.linalg.norm(r) <= tol * bnorm else total), total

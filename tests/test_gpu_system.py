@@ -1,0 +1,89 @@
+"""GPU fiber + shell system against the FP64 oracle (tstep.time_step_hydro path)."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel2
+from fiber3d_amd import configs
+from fiber3d_amd.gmres import gmres
+from fiber3d_amd.system import FiberSystem, Shell
+from oracle import tstep as OT
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(nfib, N, nper, R_ball, R_shell, seed, **kw):
+    fibers = configs.make_fibers(configs.FiberSpec(nfib, N, 1.0, R_ball), seed=seed)
+    xs, ns, ws = configs.make_periphery(nper, R_shell)
+    ora = OT.HydroSystem(fibers, (xs, ns, ws), eta=1.0, dt=1e-3, E=0.01)
+    gpu = FiberSystem(fibers, Shell(xs, ns, ws), eta=1.0, dt=1e-3, E=0.01, **kw)
+    return ora, gpu
+
+
+@pytest.fixture(scope="module")
+def small():
+    print("case system small (8 fibers x 16, shell 300)", flush=True)
+    return _build(8, 16, 300, 2.0, 3.5, 5)
+
+
+def test_assembly_and_rhs(small):
+    ora, gpu = small
+    torch.cuda.synchronize()
+    blocks = gpu.dense_blocks_host()
+    for k in range(len(blocks)):
+        # straight fibers: xsss = D3 x is rounding noise, so two FP64 assemblies
+        # differ by ~1e-11 in the tension rows; the FP64 bar is 1e-10
+        assert rel2(blocks[k], ora.A_sq[k]) < 1e-10
+    assert rel2(gpu.rhs.cpu().numpy(), ora.rhs) < 1e-12
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-10), ("f32c", 1e-6)])
+def test_matvec(small, precision, tol):
+    ora, gpu = small
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=ora.size)
+    y = gpu.matvec(torch.from_numpy(x).cuda(), precision=precision)
+    torch.cuda.synchronize()
+    assert rel2(y.cpu().numpy(), ora.matvec(x)) < tol
+
+
+def test_precond(small):
+    ora, gpu = small
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=ora.size)
+    y = gpu.precond(torch.from_numpy(x).cuda())
+    torch.cuda.synchronize()
+    assert rel2(y.cpu().numpy(), ora.precond(x)) < 1e-10
+
+
+def test_gmres_small_matches_dense_solve(small):
+    ora, gpu = small
+    xd = np.linalg.solve(ora.dense_operator(), ora.rhs)
+    res = gmres(gpu.matvec, gpu.rhs, gpu.precond, tol=1e-10)
+    torch.cuda.synchronize()
+    assert res.info == 0
+    assert rel2(res.x.cpu().numpy(), xd) < 1e-7
+
+
+def test_config0_gmres_to_1e8():
+    """Workload 0: 32 fibers x 32 nodes, 1024-node shell, one GMRES solve to 1e-8."""
+    print("case config0", flush=True)
+    cfg = configs.config(0)
+    fibers = configs.make_fibers(cfg.fibers, cfg.seed)
+    xs, ns, ws = configs.make_periphery(cfg.periphery_nodes, cfg.periphery_radius)
+    gpu = FiberSystem(fibers, Shell(xs, ns, ws), eta=cfg.eta, dt=cfg.dt, E=cfg.fibers.E)
+    res = gmres(gpu.matvec, gpu.rhs, gpu.precond, tol=1e-8, restart=150, maxiter=200)
+    torch.cuda.synchronize()
+    assert res.info == 0 and res.residual <= 1e-8
+    # the GPU solution satisfies the oracle's own system to the same tolerance
+    ora = OT.HydroSystem(fibers, (xs, ns, ws), eta=cfg.eta, dt=cfg.dt, E=cfg.fibers.E)
+    x = res.x.cpu().numpy()
+    assert rel2(gpu.rhs.cpu().numpy(), ora.rhs) < 1e-12
+    r = np.linalg.norm(ora.rhs - ora.matvec(x)) / np.linalg.norm(ora.rhs)
+    assert r <= 1.05e-8
+    # the FP32-compensated inner operator reaches the same true FP64 residual
+    res32 = gmres(gpu.matvec, gpu.rhs, gpu.precond, tol=1e-8,
+                  inner=lambda v, out=None: gpu.matvec(v, out=out, precision="f32c"))
+    torch.cuda.synchronize()
+    assert res32.info == 0 and res32.residual <= 1e-8
+    assert rel2(res32.x.cpu().numpy(), x) < 1e-6
