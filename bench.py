@@ -53,12 +53,70 @@ def cpu_baseline(x, f, n_targets=4096, cores=None):
             "sample": f"Oseen FP64, all {N_PTS} sources x {n_targets} targets, median of 3"}
 
 
+def system_leg(cfg_id, steps, train_steps=0, scale=None):
+    """time-steps/s of the implicit fiber step, plain FP64 vs fast (+ learned guess).
+
+    Each mode runs its own trajectory from the same seeded initial state:
+    `train_steps` untimed steps (the learned mode trains its predictor on them),
+    then `steps` timed steps, each solved to the configuration's GMRES
+    tolerance with the FP64 residual deciding convergence."""
+    import torch
+    from fiber3d_amd import configs
+    from fiber3d_amd.learned import LearnedGuess
+    from fiber3d_amd.system import Shell
+    from fiber3d_amd.tstep import TimeStepper
+
+    cfg = configs.config(cfg_id)
+    spec = cfg.fibers
+    nper = cfg.periphery_nodes
+    if scale is not None:
+        spec = configs.FiberSpec(scale[0], spec.num_points, spec.length, spec.ball_radius,
+                                 spec.E, spec.epsilon, spec.min_sep, spec.end_force)
+        nper = scale[1]
+    fibers = configs.make_fibers(spec, cfg.seed)
+    xs, ns, ws = configs.make_periphery(nper, cfg.periphery_radius)
+    shell = Shell(xs, ns, ws, eta=cfg.eta)
+    out = {"workload": f"{cfg.name}: {len(fibers)} fibers x {spec.num_points} nodes, "
+                       f"{nper}-node periphery, dt={cfg.dt}, GMRES tol {cfg.gmres_tol}"}
+    modes = [("plain", "plain", None), ("fast", "fast", None)]
+    if train_steps:
+        modes.append(("learned_fast", "fast", LearnedGuess(train_steps=train_steps)))
+    for name, mode, pred in modes:
+        ts = TimeStepper(fibers, shell, mode=mode, eta=cfg.eta, dt=cfg.dt, E=spec.E,
+                         epsilon=spec.epsilon, tol=cfg.gmres_tol, restart=cfg.restart,
+                         maxiter=cfg.maxiter, predictor=pred, lengths=np.full(len(fibers), spec.length))
+        warm = max(1, train_steps if pred is not None else 1)
+        for _ in range(warm):
+            ts.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its, res = [], []
+        for _ in range(steps):
+            st = ts.step()
+            its.append(st.iterations)
+            res.append(st.residual)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"time_steps_per_s": round(steps / dt, 4), "mean_gmres_iters": float(np.mean(its)),
+                     "max_true_residual": float(np.max(res)), "steps": steps, "untimed_steps": warm}
+    if "plain" in out and "fast" in out:
+        out["speedup_fast_vs_plain"] = round(out["fast"]["time_steps_per_s"] / out["plain"]["time_steps_per_s"], 3)
+    if "learned_fast" in out:
+        out["speedup_learned_fast_vs_plain"] = round(
+            out["learned_fast"]["time_steps_per_s"] / out["plain"]["time_steps_per_s"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--system", type=int, default=-1,
+                    help="also time implicit steps of fiber-system workload 0 or 3")
+    ap.add_argument("--system-steps", type=int, default=5)
+    ap.add_argument("--train-steps", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
@@ -141,6 +199,8 @@ def main():
         "stresslet_f64_gpairs_s": round(pairs / (sms64 / 1e3) / 1e9, 3),
         "stresslet_f32c_rel_err_vs_f64": serr,
     }
+    if args.system >= 0:
+        line["fiber_system"] = system_leg(args.system, args.system_steps, args.train_steps)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(x, f)
     print(json.dumps(line), flush=True)

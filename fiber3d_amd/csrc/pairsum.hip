@@ -7,17 +7,27 @@
 //
 // Design (MI355X / CDNA4, wave64):
 //   * pack_kernel converts the caller's sources (any AoS/SoA strides, FP64) into
-//     one contiguous record per source (8 or 12 reals: position, density[,
-//     normal]), multiplying an optional per-source weight, rounding to the
-//     compute type, and zero-padding to a multiple of the 64-source tile.
+//     one contiguous record per source (8 or 12 32/64-bit words: position,
+//     density[, normal]), multiplying an optional per-source weight, and
+//     zero-padding to a multiple of the 64-source tile.
 //   * pairsum_kernel: each lane owns K targets (registers).  The source index
 //     is wave-uniform, so source records are fetched with scalar loads into
 //     SGPRs and every VALU op takes its source operand straight from an SGPR:
 //     no LDS traffic and no VGPRs spent on sources.  Sources are split into
 //     P chunks over grid.y so the launch carries thousands of waves.
-//   * FP32C: pairs are summed in plain FP32 within a 64-source tile, tile sums
-//     are added with Kahan compensation, and (sum, -compensation) are written
-//     as FP32 partials per chunk; reduce_kernel combines chunks in FP64.
+//   * FP32C: positions are quantised once per call to 31-bit integers on the
+//     bounding box of sources and targets (bbox_kernel), so d = t - s is an
+//     exact integer difference rounded once to FP32; the pair is then
+//     evaluated in FP32 in units of the quantum h and the sum rescaled by
+//     h^-p (the kernels are homogeneous of degree -1 / -2 in d).  Rounding the
+//     coordinates themselves to FP32 would cost ~1e-7 |x| / r relative per
+//     pair, which fails 1e-6 for the close pairs of random point sets.
+//     Pairs are summed in plain FP32 within a 64-source tile, tile sums are
+//     added with Kahan compensation, and (sum, -compensation) are written as
+//     FP32 partials per chunk; reduce_kernel combines chunks in FP64.
+//   * Near pairs: the regularisation test r < eps is one compare per pair
+//     whose result is OR-ed over the tile; only a tile in which some lane saw
+//     a near pair is recomputed with the exact regularised form.
 //   * Exclusion ranges [lo_i, hi_i) (same fiber / self term) are tested only in
 //     tiles that intersect the wave's union of ranges; all other tiles run a
 //     mask-free path.
@@ -26,15 +36,24 @@
 #include <stdarg.h>
 #include <stdlib.h>
 
+#include <type_traits>
+
 namespace f3d {
 
 constexpr int kTile = 64;        // sources per compensated tile
 constexpr int kBlock = 256;      // threads per block (4 waves)
 constexpr int kPackBlocks = 256; // grid of the pack kernel (sigma partials)
+constexpr double kQuantLevels = 2147483648.0 - 65536.0;  // 2^31 minus margin
 
 template <int KIND>
 struct RecLen {
   static constexpr int value = (KIND == F3D_STRESSLET) ? 12 : 8;
+};
+
+// homogeneity degree of the kernel in d (for the FP32C rescaling)
+template <int KIND>
+struct Degree {
+  static constexpr int value = (KIND == F3D_OSEEN) ? 1 : 2;
 };
 
 __device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -47,24 +66,97 @@ __device__ __forceinline__ double rsqrt_t(double x) {
   return fma(0.5 * y, e, y);
 }
 
+// Quantisation geometry of one FP32C call, computed on the device:
+// q = rint((x - origin) / h), h = extent / kQuantLevels.
+struct Geo {
+  double origin[3];
+  double h;
+  double inv_h;
+  double pad[3];
+};
+
+// ---------------------------------------------------------------------------
+// Bounding box of sources and targets -> Geo (one workgroup).
+__global__ void __launch_bounds__(1024)
+bbox_kernel(int ns, const double* __restrict__ pos, long sp, long sc, int nt,
+            const double* __restrict__ tpos, long t_sp, long t_sc, double min_extent,
+            Geo* __restrict__ geo) {
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int j = threadIdx.x; j < ns + nt; j += blockDim.x) {
+    const double* p = j < ns ? pos + (long)j * sp : tpos + (long)(j - ns) * t_sp;
+    const long c = j < ns ? sc : t_sc;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double v = p[a * c];
+      lo[a] = fmin(lo[a], v);
+      hi[a] = fmax(hi[a], v);
+    }
+  }
+  __shared__ double red[16][6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      lo[a] = fmin(lo[a], __shfl_xor(lo[a], o, 64));
+      hi[a] = fmax(hi[a], __shfl_xor(hi[a], o, 64));
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      red[w][a] = lo[a];
+      red[w][3 + a] = hi[a];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    double ext = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      double l = red[0][a], u = red[0][3 + a];
+      for (int k = 1; k < nw; ++k) {
+        l = fmin(l, red[k][a]);
+        u = fmax(u, red[k][3 + a]);
+      }
+      geo->origin[a] = l;
+      ext = fmax(ext, u - l);
+    }
+    ext = fmax(ext, min_extent);
+    geo->h = ext / kQuantLevels;
+    geo->inv_h = kQuantLevels / ext;
+  }
+}
+
+__device__ __forceinline__ int quantise(double x, double origin, double inv_h) {
+  return (int)rint((x - origin) * inv_h);
+}
+
 // ---------------------------------------------------------------------------
 // pack: sources -> records.  Also accumulates sigma = sum_j n_j . q_j per
 // block when sigma_part != nullptr (periphery complementary term).
+// FP32C records hold the quantised position as int32 bits in words 0..2.
 template <typename T, int KIND>
 __global__ void __launch_bounds__(256)
 pack_kernel(int ns, int ns_pad, const double* __restrict__ pos, const double* __restrict__ nrm,
             const double* __restrict__ den, const double* __restrict__ w, long sp, long sc,
-            T* __restrict__ rec, double* __restrict__ sigma_part) {
+            const Geo* __restrict__ geo, T* __restrict__ rec, double* __restrict__ sigma_part) {
   constexpr int R = RecLen<KIND>::value;
+  constexpr bool kQuant = std::is_same<T, float>::value;
   double sig = 0.0;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ns_pad; j += gridDim.x * blockDim.x) {
     T* r = rec + (size_t)j * R;
     if (j < ns) {
       const double wj = w ? w[j] : 1.0;
       const long b = (long)j * sp;
-      r[0] = (T)pos[b];
-      r[1] = (T)pos[b + sc];
-      r[2] = (T)pos[b + 2 * sc];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double x = pos[b + a * sc];
+        if constexpr (kQuant)
+          r[a] = __int_as_float(quantise(x, geo->origin[a], geo->inv_h));
+        else
+          r[a] = x;
+      }
       if (KIND == F3D_STRESSLET) {
         const double n0 = nrm[b], n1 = nrm[b + sc], n2 = nrm[b + 2 * sc];
         const double q0 = den[b] * wj, q1 = den[b + sc] * wj, q2 = den[b + 2 * sc] * wj;
@@ -79,8 +171,9 @@ pack_kernel(int ns, int ns_pad, const double* __restrict__ pos, const double* __
         r[6] = 0; r[7] = 0;
       }
     } else {
-      // Zero-density padding: contributes exactly zero (finite r since the
-      // regularisation keeps r >= reg when a target sits at the origin).
+      // Zero-density padding contributes exactly zero: its r is finite because
+      // the near-pair regularisation keeps r >= reg (a target sitting at the
+      // padding position is regularised like any coincident pair).
 #pragma unroll
       for (int c = 0; c < R; ++c) r[c] = 0;
     }
@@ -95,19 +188,35 @@ pack_kernel(int ns, int ns_pad, const double* __restrict__ pos, const double* __
 }
 
 // ---------------------------------------------------------------------------
+// Target coordinate type: quantised int32 for FP32C, FP64 for F64.
+template <typename T>
+using TCoord = typename std::conditional<std::is_same<T, float>::value, int, double>::type;
+
+// near-pair handling of one pair
+enum { MODE_EXACT = 0, MODE_CLAMP = 1, MODE_VOTE = 2 };
+
 // One pair: accumulate the un-scaled kernel contribution of record r into
-// (ax, ay, az).  MASK: zero the pair when excl is set.  CLAMP: cap 1/r at
-// rmax instead of the exact regularisation.
-template <typename T, int KIND, bool CLAMP, bool MASK>
-__device__ __forceinline__ void pair(const T* r, T tx, T ty, T tz, T reg2, T eps2,
-                                     T rmax, bool excl, T& ax, T& ay, T& az) {
-  const T dx = tx - r[0];
-  const T dy = ty - r[1];
-  const T dz = tz - r[2];
+// (ax, ay, az).  MASK: zero the pair when excl is set.
+template <typename T, int KIND, int MODE, bool MASK>
+__device__ __forceinline__ void pair(const T* r, TCoord<T> tx, TCoord<T> ty, TCoord<T> tz,
+                                     T reg2, T eps2, T rmax, bool excl, T& ax, T& ay, T& az,
+                                     unsigned long long& vote) {
+  T dx, dy, dz;
+  if constexpr (std::is_same<T, float>::value) {
+    dx = (float)(tx - __float_as_int(r[0]));
+    dy = (float)(ty - __float_as_int(r[1]));
+    dz = (float)(tz - __float_as_int(r[2]));
+  } else {
+    dx = tx - r[0];
+    dy = ty - r[1];
+    dz = tz - r[2];
+  }
   T r2 = dx * dx + dy * dy + dz * dz;
-  if (!CLAMP) r2 = (r2 < eps2) ? r2 + reg2 : r2;
+  if (MODE == MODE_EXACT) r2 = (r2 < eps2) ? r2 + reg2 : r2;
+  // the near-pair vote lives in SGPRs: one v_cmp into a lane mask, OR-ed on the SALU
+  if (MODE == MODE_VOTE) vote |= __ballot(r2 < eps2);
   T ri = rsqrt_t(r2);
-  if (CLAMP) ri = ri < rmax ? ri : rmax;
+  if (MODE == MODE_CLAMP) ri = ri < rmax ? ri : rmax;
   if (MASK) ri = excl ? (T)0 : ri;
   if (KIND == F3D_OSEEN) {
     const T fd = dx * r[3] + dy * r[4] + dz * r[5];
@@ -131,14 +240,17 @@ __device__ __forceinline__ void pair(const T* r, T tx, T ty, T tz, T reg2, T eps
   }
 }
 
-template <typename T, int KIND, int K, bool CLAMP, bool MASK>
-__device__ __forceinline__ void tile(const T* __restrict__ rec, int j0, const T (&tx)[K],
-                                     const T (&ty)[K], const T (&tz)[K], const int (&lo)[K],
-                                     const int (&hi)[K], T reg2, T eps2, T rmax, T (&ax)[K],
-                                     T (&ay)[K], T (&az)[K]) {
+// One 64-source tile; returns (wave-uniformly) whether any pair was near when
+// MODE == MODE_VOTE.
+template <typename T, int KIND, int K, int MODE, bool MASK>
+__device__ __forceinline__ bool tile(const T* __restrict__ rec, int j0, const TCoord<T> (&tx)[K],
+                                     const TCoord<T> (&ty)[K], const TCoord<T> (&tz)[K],
+                                     const int (&lo)[K], const int (&hi)[K], T reg2, T eps2,
+                                     T rmax, T (&ax)[K], T (&ay)[K], T (&az)[K]) {
   constexpr int R = RecLen<KIND>::value;
   constexpr int U = 4;  // sources whose records are fetched together
   constexpr int NV = (KIND == F3D_STRESSLET) ? 9 : 6;
+  unsigned long long vote = 0;
   for (int jj = 0; jj < kTile; jj += U) {
     // Issue the U scalar record loads before any use so their latencies overlap.
     T s[U][NV];
@@ -154,11 +266,12 @@ __device__ __forceinline__ void tile(const T* __restrict__ rec, int j0, const T 
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const bool excl = MASK && ((unsigned)(j - lo[k]) < (unsigned)(hi[k] - lo[k]));
-        pair<T, KIND, CLAMP, MASK>(s[u], tx[k], ty[k], tz[k], reg2, eps2, rmax, excl, ax[k],
-                                   ay[k], az[k]);
+        pair<T, KIND, MODE, MASK>(s[u], tx[k], ty[k], tz[k], reg2, eps2, rmax, excl, ax[k],
+                                  ay[k], az[k], vote);
       }
     }
   }
+  return MODE == MODE_VOTE && vote != 0ull;
 }
 
 // partial layout: part[(p*3 + c)*nt + i]; for T=float a second array holds
@@ -167,10 +280,16 @@ template <typename T, int KIND, int K, bool CLAMP>
 __global__ void __launch_bounds__(kBlock)
 pairsum_kernel(const T* __restrict__ rec, int chunk, int ns_pad, const double* __restrict__ tpos,
                long t_sp, long t_sc, int nt, const int* __restrict__ excl_lo,
-               const int* __restrict__ excl_hi, T reg2, T eps2, T rmax, T* __restrict__ part_hi,
-               T* __restrict__ part_lo) {
+               const int* __restrict__ excl_hi, double reg, double eps,
+               const Geo* __restrict__ geo, T* __restrict__ part_hi, T* __restrict__ part_lo) {
+  constexpr bool kQuant = std::is_same<T, float>::value;
+  // regularisation constants in the units of the coordinates used
+  const double hs = kQuant ? geo->h : 1.0;
+  const T reg2 = (T)((reg / hs) * (reg / hs));
+  const T eps2 = (T)((eps / hs) * (eps / hs));
+  const T rmax = (T)(hs / eps);
   const int base = blockIdx.x * (kBlock * K) + threadIdx.x;
-  T tx[K], ty[K], tz[K];
+  TCoord<T> tx[K], ty[K], tz[K];
   int lo[K], hi[K];
   int wlo = 0x7fffffff, whi = -0x7fffffff;
 #pragma unroll
@@ -178,9 +297,15 @@ pairsum_kernel(const T* __restrict__ rec, int chunk, int ns_pad, const double* _
     const int i = base + k * kBlock;
     const int ic = i < nt ? i : nt - 1;
     const long b = (long)ic * t_sp;
-    tx[k] = (T)tpos[b];
-    ty[k] = (T)tpos[b + t_sc];
-    tz[k] = (T)tpos[b + 2 * t_sc];
+    if constexpr (kQuant) {
+      tx[k] = quantise(tpos[b], geo->origin[0], geo->inv_h);
+      ty[k] = quantise(tpos[b + t_sc], geo->origin[1], geo->inv_h);
+      tz[k] = quantise(tpos[b + 2 * t_sc], geo->origin[2], geo->inv_h);
+    } else {
+      tx[k] = tpos[b];
+      ty[k] = tpos[b + t_sc];
+      tz[k] = tpos[b + 2 * t_sc];
+    }
     lo[k] = 0;
     hi[k] = 0;
     if (excl_lo != nullptr && i < nt) {
@@ -206,10 +331,24 @@ pairsum_kernel(const T* __restrict__ rec, int chunk, int ns_pad, const double* _
     T ax[K], ay[K], az[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = 0;
-    if (j0 < whi && j0 + kTile > wlo)
-      tile<T, KIND, K, CLAMP, true>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax, ay, az);
-    else
-      tile<T, KIND, K, CLAMP, false>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax, ay, az);
+    if (j0 < whi && j0 + kTile > wlo) {
+      if (CLAMP)
+        tile<T, KIND, K, MODE_CLAMP, true>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax,
+                                           ay, az);
+      else
+        tile<T, KIND, K, MODE_EXACT, true>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax,
+                                           ay, az);
+    } else if (CLAMP) {
+      tile<T, KIND, K, MODE_CLAMP, false>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax, ay,
+                                          az);
+    } else if (tile<T, KIND, K, MODE_VOTE, false>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax,
+                                                  ax, ay, az)) {
+      // some lane met a pair with r < eps: redo the tile with the regularised form
+#pragma unroll
+      for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = 0;
+      tile<T, KIND, K, MODE_EXACT, false>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax, ay,
+                                          az);
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (sizeof(T) == 4) {
@@ -257,8 +396,9 @@ struct ShellEpi {
 template <typename T>
 __global__ void __launch_bounds__(256)
 reduce_kernel(const T* __restrict__ part_hi, const T* __restrict__ part_lo, int P, int nt,
-              double factor, double* __restrict__ out, long o_sp, long o_sc, int accumulate,
-              ShellEpi epi) {
+              double factor, const Geo* __restrict__ geo, int degree, double* __restrict__ out,
+              long o_sp, long o_sc, int accumulate, ShellEpi epi) {
+  if (geo) factor *= degree == 1 ? geo->inv_h : geo->inv_h * geo->inv_h;
   double sigma = 0.0;
   if (epi.sigma_part) {
     __shared__ double red[4];
@@ -348,7 +488,7 @@ static size_t part_bytes(int precision, int P, int64_t nt) {
 static size_t pairsum_ws(int kernel, int precision, int64_t ns, int64_t nt) {
   const Plan pl = make_plan(precision, ns, nt);
   return rec_bytes(kernel, precision, pl.ns_pad) + part_bytes(precision, pl.P, nt) +
-         align_up(kPackBlocks * sizeof(double), 256);
+         align_up(kPackBlocks * sizeof(double), 256) + align_up(sizeof(Geo), 256);
 }
 
 template <typename T, int KIND>
@@ -358,26 +498,34 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
                           int64_t t_sc, const int32_t* lo, const int32_t* hi, double eta,
                           double reg, double eps, double* out, int64_t o_sp, int64_t o_sc,
                           char* ws, const ShellEpi* epi_in, hipStream_t st) {
+  constexpr bool kQuant = std::is_same<T, float>::value;
+  const int prec = kQuant ? F3D_F32C : F3D_F64;
   T* rec = reinterpret_cast<T*>(ws);
-  char* p = ws + rec_bytes(KIND, sizeof(T) == 8 ? F3D_F64 : F3D_F32C, pl.ns_pad);
+  char* p = ws + rec_bytes(KIND, prec, pl.ns_pad);
   T* part_hi = reinterpret_cast<T*>(p);
   T* part_lo = part_hi + (size_t)pl.P * 3 * nt;
-  double* sigma_part = reinterpret_cast<double*>(
-      p + part_bytes(sizeof(T) == 8 ? F3D_F64 : F3D_F32C, pl.P, nt));
+  p += part_bytes(prec, pl.P, nt);
+  double* sigma_part = reinterpret_cast<double*>(p);
+  p += align_up(kPackBlocks * sizeof(double), 256);
+  Geo* geo = kQuant ? reinterpret_cast<Geo*>(p) : nullptr;
 
+  if (kQuant) {
+    // the quantum must resolve eps; an extent below 64 (eps + reg) is raised
+    const double min_extent = 64.0 * (eps + reg) + 1e-300;
+    F3D_LAUNCH(bbox_kernel, dim3(1), dim3(1024), 0, st, (int)ns, pos, (long)s_sp, (long)s_sc,
+               (int)nt, tpos, (long)t_sp, (long)t_sc, min_extent, geo);
+  }
   const bool want_sigma = epi_in != nullptr;
   F3D_LAUNCH((pack_kernel<T, KIND>), dim3(kPackBlocks), dim3(256), 0, st, (int)ns, pl.ns_pad,
-             pos, nrm, den, w, (long)s_sp, (long)s_sc, rec, want_sigma ? sigma_part : nullptr);
+             pos, nrm, den, w, (long)s_sp, (long)s_sc, (const Geo*)geo, rec,
+             want_sigma ? sigma_part : nullptr);
 
-  const T reg2 = (T)(reg * reg);
-  const T eps2 = (T)(eps * eps);
-  const T rmax = (T)(1.0 / eps);
   const dim3 grid((unsigned)ceil_div(nt, (int64_t)kBlock * pl.K), (unsigned)pl.P);
-  const bool clamp = (sizeof(T) == 4) && (flags & F3D_CLAMP_NEAR);
+  const bool clamp = kQuant && (flags & F3D_CLAMP_NEAR);
 #define F3D_PS(KK, CL)                                                                        \
   F3D_LAUNCH((pairsum_kernel<T, KIND, KK, CL>), grid, dim3(kBlock), 0, st, rec, pl.chunk,     \
-             pl.ns_pad, tpos, (long)t_sp, (long)t_sc, (int)nt, lo, hi, reg2, eps2, rmax,      \
-             part_hi, part_lo)
+             pl.ns_pad, tpos, (long)t_sp, (long)t_sc, (int)nt, lo, hi, reg, eps,              \
+             (const Geo*)geo, part_hi, part_lo)
   if (clamp) {
     if (pl.K == 1) F3D_PS(1, true);
     else if (pl.K == 2) F3D_PS(2, true);
@@ -398,8 +546,8 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
     epi.sigma_part = sigma_part;
   }
   F3D_LAUNCH((reduce_kernel<T>), dim3((unsigned)ceil_div(nt, 256)), dim3(256), 0, st, part_hi,
-             part_lo, pl.P, (int)nt, factor, out, (long)o_sp, (long)o_sc,
-             (flags & F3D_ACCUMULATE) ? 1 : 0, epi);
+             part_lo, pl.P, (int)nt, factor, (const Geo*)geo, Degree<KIND>::value, out,
+             (long)o_sp, (long)o_sc, (flags & F3D_ACCUMULATE) ? 1 : 0, epi);
   return F3D_OK;
 }
 
@@ -427,8 +575,8 @@ static int pairsum_impl(int kernel, int precision, uint32_t flags, int64_t ns, c
     if (flags & F3D_ACCUMULATE) return F3D_OK;
     // out = 0 on the caller's layout: reuse the reduce kernel with P = 0.
     F3D_LAUNCH((reduce_kernel<double>), dim3((unsigned)ceil_div(nt, 256)), dim3(256), 0, st,
-               (const double*)nullptr, (const double*)nullptr, 0, (int)nt, 0.0, out, (long)o_sp,
-               (long)o_sc, 0, ShellEpi{});
+               (const double*)nullptr, (const double*)nullptr, 0, (int)nt, 0.0,
+               (const Geo*)nullptr, 1, out, (long)o_sp, (long)o_sc, 0, ShellEpi{});
     return F3D_OK;
   }
   F3D_REQUIRE(pos != nullptr && den != nullptr, F3D_EINVAL, "pairsum: NULL source arrays");
