@@ -107,6 +107,60 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None):
     return out
 
 
+def lu_leg(reps=5, cpu=True):
+    """Batched fiber preconditioner at workload 2: 2048 fibers, N in
+    {16,24,32,48,64} (block order n = 4N), A_sq from the fiber discretisation.
+    One persistent factor launch (FP64, or FP32 + FP64 refinement) and one
+    solve launch for all fibers."""
+    import torch
+    from fiber3d_amd import configs
+    from fiber3d_amd.dense import BatchedLU
+    from fiber3d_amd.system import FiberSystem
+
+    cfg = configs.config(2)
+    fibers = configs.make_fibers(cfg.fibers, cfg.seed)
+    S = FiberSystem(fibers, None, lu_precision="f64")
+    n = S.bs.sizes.astype(np.float64)
+    flops = float(np.sum(2.0 / 3.0 * n**3))
+    rng = np.random.default_rng(2)
+    xs = torch.from_numpy(rng.normal(size=S.bs.nrows)).cuda()
+    b = torch.empty_like(xs)
+    S.gA(S.A_buf, xs, b)
+    out = {"workload": f"{len(fibers)} fibers, N in {cfg.fibers.num_points}, n = 4N",
+           "factor_flops": flops}
+    for prec in ("f64", "f32c"):
+        lu = BatchedLU(S.bs, S.A_buf, precision=prec, refine=10, tol=1e-12)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lu.factor()
+        torch.cuda.synchronize()
+        tf = (time.perf_counter() - t0) / reps
+        y = lu.solve(b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            y = lu.solve(b)
+        torch.cuda.synchronize()
+        ts = (time.perf_counter() - t0) / reps
+        err = float(torch.linalg.norm(y - xs) / torch.linalg.norm(xs))
+        out[prec] = {"factor_ms": round(tf * 1e3, 3), "factor_gflops": round(flops / tf / 1e9, 1),
+                     "solve_ms": round(ts * 1e3, 3), "forward_error": err,
+                     "max_refine_sweeps": int(lu.iters.max()) if prec == "f32c" else 0}
+    if cpu:
+        import scipy.linalg as sla
+        blocks = S.bs.unpack(S.A_buf)[:256]
+        t0 = time.perf_counter()
+        for A in blocks:
+            sla.lu_factor(A, check_finite=False)
+        dt = time.perf_counter() - t0
+        nb = np.array([A.shape[0] for A in blocks], float)
+        out["cpu_baseline"] = {"kind": "library (SciPy/LAPACK dgetrf per fiber, host)",
+                               "sample": "first 256 of the fibers", "factor_gflops":
+                               round(float(np.sum(2 / 3 * nb**3)) / dt / 1e9, 2)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +171,7 @@ def main():
                     help="also time implicit steps of fiber-system workload 0 or 3")
     ap.add_argument("--system-steps", type=int, default=5)
     ap.add_argument("--train-steps", type=int, default=0)
+    ap.add_argument("--lu", action="store_true", help="also time the batched fiber LU (workload 2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
@@ -199,6 +254,8 @@ def main():
         "stresslet_f64_gpairs_s": round(pairs / (sms64 / 1e3) / 1e9, 3),
         "stresslet_f32c_rel_err_vs_f64": serr,
     }
+    if args.lu:
+        line["batched_lu"] = lu_leg(cpu=not args.no_cpu)
     if args.system >= 0:
         line["fiber_system"] = system_leg(args.system, args.system_steps, args.train_steps)
     if not args.no_cpu:

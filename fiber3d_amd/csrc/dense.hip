@@ -388,6 +388,7 @@ lu_solve_kernel(const int* __restrict__ n_arr, const long* __restrict__ a_off,
 
   int it = 0;
   double anorm = -1.0;
+  double rprev = 0.0;
   for (; it < refine; ++it) {
     // r = x - A y in FP64 (first sweep also measures ||A||_inf)
     double r[kSlots], rowabs[kSlots];
@@ -427,6 +428,9 @@ lu_solve_kernel(const int* __restrict__ n_arr, const long* __restrict__ a_off,
     yn = wave_max_f64(yn);
     if (anorm < 0.0) anorm = wave_max_f64(an);
     if (rn <= tol * anorm * yn) break;
+    // stagnation: FP64 refinement stopped contracting the residual
+    if (it > 0 && rn > 0.5 * rprev) break;
+    rprev = rn;
     // dy = solve(r)
 #pragma unroll
     for (int m = 0; m < kSlots; ++m) {

@@ -457,13 +457,22 @@ static int env_int(const char* name, int dflt) {
   return atoi(s);
 }
 
-static Plan make_plan(int precision, int64_t ns, int64_t nt) {
+// Launch shape per (kernel, precision): targets per lane and the wave count
+// the source split aims for.  Set by the autotuner through
+// f3d_pairsum_set_launch; F3D_PAIRSUM_K32 / _K64 / _WAVES override.
+static int g_launch_k[3][2] = {{2, 2}, {2, 2}, {2, 2}};
+static int g_launch_waves[3][2] = {{8192, 8192}, {8192, 8192}, {8192, 8192}};
+
+static Plan make_plan(int kernel, int precision, int64_t ns, int64_t nt) {
   Plan pl;
-  pl.K = env_int(precision == F3D_F64 ? "F3D_PAIRSUM_K64" : "F3D_PAIRSUM_K32", 2);
+  const int kk = (kernel >= 0 && kernel < 3) ? kernel : 0;
+  const int pp = precision == F3D_F64 ? 0 : 1;
+  pl.K = env_int(precision == F3D_F64 ? "F3D_PAIRSUM_K64" : "F3D_PAIRSUM_K32", g_launch_k[kk][pp]);
   if (pl.K != 1 && pl.K != 2 && pl.K != 4) pl.K = 2;
   pl.ns_pad = (int)align_up((size_t)(ns > 0 ? ns : 1), kTile);
   const int64_t waves_per_split = ceil_div(nt > 0 ? nt : 1, 64 * pl.K);
-  const int target_waves = env_int("F3D_PAIRSUM_WAVES", 8192);
+  int target_waves = env_int("F3D_PAIRSUM_WAVES", g_launch_waves[kk][pp]);
+  if (target_waves < 64) target_waves = 64;
   const int64_t tiles = pl.ns_pad / kTile;
   int64_t P = ceil_div(target_waves, waves_per_split);
   if (P < 1) P = 1;
@@ -486,7 +495,7 @@ static size_t part_bytes(int precision, int P, int64_t nt) {
 }
 
 static size_t pairsum_ws(int kernel, int precision, int64_t ns, int64_t nt) {
-  const Plan pl = make_plan(precision, ns, nt);
+  const Plan pl = make_plan(kernel, precision, ns, nt);
   return rec_bytes(kernel, precision, pl.ns_pad) + part_bytes(precision, pl.P, nt) +
          align_up(kPackBlocks * sizeof(double), 256) + align_up(sizeof(Geo), 256);
 }
@@ -585,7 +594,7 @@ static int pairsum_impl(int kernel, int precision, uint32_t flags, int64_t ns, c
   const size_t need = pairsum_ws(kernel, precision, ns, nt);
   F3D_REQUIRE(ws != nullptr && ws_bytes >= need, F3D_EWORKSPACE,
               "pairsum: workspace %zu bytes < required %zu", ws_bytes, need);
-  const Plan pl = make_plan(precision, ns, nt);
+  const Plan pl = make_plan(kernel, precision, ns, nt);
   char* wsc = reinterpret_cast<char*>(ws);
 #define F3D_ARGS pl, flags, ns, pos, nrm, den, w, s_sp, s_sc, nt, tpos, t_sp, t_sc, lo, hi, eta, \
                  reg, eps, out, o_sp, o_sc, wsc, epi, st
@@ -604,6 +613,21 @@ static int pairsum_impl(int kernel, int precision, uint32_t flags, int64_t ns, c
 }  // namespace f3d
 
 using namespace f3d;
+
+extern "C" int f3d_pairsum_set_launch(int kernel, int precision, int32_t k,
+                                      int32_t target_waves) {
+  F3D_REQUIRE(kernel == F3D_OSEEN || kernel == F3D_ROTLET || kernel == F3D_STRESSLET, F3D_EINVAL,
+              "set_launch: unknown kernel %d", kernel);
+  F3D_REQUIRE(precision == F3D_F64 || precision == F3D_F32C, F3D_EINVAL,
+              "set_launch: unknown precision %d", precision);
+  F3D_REQUIRE(k == 0 || k == 1 || k == 2 || k == 4, F3D_EINVAL, "set_launch: K must be 1, 2 or 4");
+  F3D_REQUIRE(target_waves == 0 || (target_waves >= 64 && target_waves <= (1 << 20)), F3D_EINVAL,
+              "set_launch: target_waves outside [64, 2^20]");
+  const int pp = precision == F3D_F64 ? 0 : 1;
+  g_launch_k[kernel][pp] = k ? k : 2;
+  g_launch_waves[kernel][pp] = target_waves ? target_waves : 8192;
+  return F3D_OK;
+}
 
 extern "C" size_t f3d_pairsum_workspace_size(int kernel, int precision, int64_t ns, int64_t nt) {
   if (precision != F3D_F64 && precision != F3D_F32C) return 0;

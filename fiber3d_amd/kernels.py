@@ -35,6 +35,18 @@ def device():
     return torch.device("cuda")
 
 
+_tuned = False
+
+
+def _ensure_tuned():
+    """Install the saved launch-shape table (fiber3d_amd/tuning.json) once."""
+    global _tuned
+    if not _tuned:
+        _tuned = True
+        from . import autotune
+        autotune.apply()
+
+
 def workspace(nbytes, tag="default"):
     """A cached device byte buffer of at least nbytes (grown on demand)."""
     key = (tag, torch.cuda.current_device())
@@ -92,6 +104,7 @@ def pairsum(kernel, src_pos, src_den, trg_pos, src_nrm=None, src_w=None, excl=No
     """
     k = {"oseen": L.OSEEN, "rotlet": L.ROTLET, "stresslet": L.STRESSLET}[kernel]
     lib = L.load()
+    _ensure_tuned()
     (sp_, sd_, sn_), (s_sp, s_sc) = _as_common([src_pos, src_den, src_nrm])
     nt = trg_pos.shape[0]
     ns = sp_.shape[0]
@@ -121,6 +134,7 @@ def periphery_apply(pos, nrm, w, ex, ey, ez, q, add=None, eta=1.0, reg=5e-3, eps
                     precision="f64", out=None):
     """Matrix-free shell operator (f3d_periphery_apply) on (N,3) CUDA tensors."""
     lib = L.load()
+    _ensure_tuned()
     (pos, nrm, ex, ey, ez, q, add), (sp, sc) = _as_common([pos, nrm, ex, ey, ez, q, add])
     n = pos.shape[0]
     if out is None:

@@ -11,11 +11,8 @@ coefficients of the density change in a frame built from local vectors).
 Forward and backward products run on MFMA in BF16 with FP32 accumulation
 (libf3d f3d_gemm_bf16); master weights and the Adam update are FP32.
 
-Guess assembly per step: positions extrapolated linearly from the last two
-steps, tension = T_{n-1} + net(fiber features), shell density =
-q_{n-1} + sum_k c_k u_k with the net giving the c_k and u_k = (q_{n-1},
-v_n, n (n . v_n)) the previous density, the explicit fiber flow on the shell
-node and its normal component.
+Guess assembly per step: the previous two solutions extrapolated linearly,
+corrected by the nets (see LearnedGuess).
 """
 import numpy as np
 import torch
@@ -128,115 +125,152 @@ class Normalizer:
 
 
 class LearnedGuess:
-    """Initial-guess predictor for TimeStepper (fiber3d_amd.tstep)."""
+    """Initial-guess predictor for TimeStepper (fiber3d_amd.tstep).
+
+    Baseline guess: the solution extrapolated linearly from the last two steps
+    (positions, tensions and shell density alike).  Two MLPs correct it: one
+    per fiber node predicts the tension correction T_n - T_ext, one per shell
+    node predicts coefficients c_k of the density correction q_n - q_ext =
+    sum_k c_k u_k in a local basis u = (dq, d2q, n (n . dq)) built from the
+    density history.  Inputs are rotation-invariant scalars of the local
+    geometry, the explicit flow and the solution history.  The last layer
+    starts at zero, so an untrained net reproduces the extrapolation, and after
+    training each net is kept only if it beats the extrapolation on held-out
+    steps (validation MSE below `gate` times the extrapolation's).
+    """
 
     def __init__(self, train_steps=100, epochs=200, lr=1e-3, batch=8192, seed=0,
-                 use_net=True):
+                 use_net=True, gate=0.9, val_frac=0.2):
         self.train_steps = train_steps
         self.epochs, self.lr, self.batch, self.seed = epochs, lr, batch, seed
         self.use_net = use_net
+        self.gate, self.val_frac = gate, val_frac
         self.samples_f = []
         self.samples_s = []
-        self.prev = []  # last solutions
+        self.prev = []  # last three solutions
         self.net_f = self.net_s = None
+        self.enabled_f = self.enabled_s = False
         self.trained = False
-        self.losses = []
+        self.report = {}
 
     # ---- features ---------------------------------------------------------
+    def _hist(self, ts, idx):
+        """T (or any gathered component) at the last three solutions."""
+        return [p[idx] for p in self.prev[-3:]]
+
     def _fiber_features(self, ts):
         S = ts.sys
-        dev = S.dev
-        feats = []
-        v = self._explicit_flow(ts)
+        v = S.v_explicit
+        T1, T2, T3 = self._hist(ts, ts.idx_t)
+        out = torch.empty((S.npts, 10), dtype=torch.float64, device=S.dev)
         for N, (node, X, xs, xss, D1, P, B) in S._ctx.items():
             vk = v[node]
+            nf = node.reshape(-1)
             alpha = S.buckets[N]["alpha"][None, :].expand(B, N)
             dv = torch.matmul(D1, vk)
-            T = ts.tension[node]
-            f = torch.stack([alpha, alpha.abs(), T, (xs * dv).sum(-1), (xss * vk).sum(-1),
-                             (xss * xss).sum(-1), (xs * vk).sum(-1), vk.norm(dim=-1)], dim=-1)
-            feats.append((node.reshape(-1), f.reshape(-1, f.shape[-1])))
-        out = torch.empty((S.npts, feats[0][1].shape[1]), dtype=torch.float64, device=dev)
-        for node, f in feats:
-            out[node] = f
+            t1, t2, t3 = T1[node], T2[node], T3[node]
+            f = torch.stack([alpha, alpha.abs(), t1, t1 - t2, t1 - 2 * t2 + t3,
+                             (xs * dv).sum(-1), (xss * vk).sum(-1), (xss * xss).sum(-1),
+                             (xs * vk).sum(-1), vk.norm(dim=-1)], dim=-1)
+            out[nf] = f.reshape(-1, 10)
         return out
 
-    def _explicit_flow(self, ts):
+    def _shell_terms(self, ts):
         S = ts.sys
-        # the per-step RHS carries the explicit flows; rebuild v on fiber nodes
-        # from the force used by the RHS (FP64 pair sum, same-fiber exclusion)
-        from .kernels import pairsum
-        return pairsum("oseen", S.X, S.force, S.X, src_w=S.weights, excl=S.excl, eta=S.eta,
-                       reg=S.reg, eps=S.eps)
-
-    def _shell_basis(self, ts):
-        S = ts.sys
-        q = ts.shell_density.view(S.nper, 3)
-        v = -S.rhs[:3 * S.nper].view(S.nper, 3)  # explicit fiber flow on the shell
+        q1, q2, q3 = (p[:3 * S.nper].view(S.nper, 3) for p in self.prev[-3:])
+        dq = q1 - q2
+        d2q = dq - (q2 - q3)
         n = S.shell.nrm
-        u = torch.stack([q, v, n * (n * v).sum(-1, keepdim=True)], dim=1)  # (nper, 3, 3)
-        inv = torch.stack([(q * q).sum(-1).sqrt(), (v * v).sum(-1).sqrt(), (n * v).sum(-1),
-                           (n * q).sum(-1)], dim=-1)
+        v = -S.rhs[:3 * S.nper].view(S.nper, 3)  # explicit fiber flow on the shell
+        u = torch.stack([dq, d2q, n * (n * dq).sum(-1, keepdim=True)], dim=1)  # (nper, 3, 3)
+        a, b = dq.norm(dim=-1), d2q.norm(dim=-1)
+        inv = torch.stack([a, b, (n * dq).sum(-1), (n * d2q).sum(-1),
+                           (dq * d2q).sum(-1) / (a * b + 1e-300), v.norm(dim=-1),
+                           (n * v).sum(-1)], dim=-1)
         return u, inv
+
+    def _extrapolated(self):
+        return 2.0 * self.prev[-1] - self.prev[-2]
 
     # ---- predictor protocol ------------------------------------------------
     def predict(self, ts):
         S = ts.sys
+        self._pending = None
         if len(self.prev) < 2:
-            return None
-        x0 = self.prev[-1].clone()
-        # positions: linear extrapolation of the last two solutions
-        x_idx = ts.idx_x.reshape(-1)
-        x0[x_idx] = 2.0 * self.prev[-1][x_idx] - self.prev[-2][x_idx]
-        if self.trained and self.use_net:
+            return self.prev[-1].clone() if self.prev else None
+        x0 = self._extrapolated()
+        if len(self.prev) < 3:
+            return x0
+        ff = self._fiber_features(ts)
+        sh = self._shell_terms(ts) if S.nper else None
+        if not self.trained:
+            self._pending = (ff, sh, x0)
+        elif self.use_net:
             with torch.no_grad():
-                ff = self.nf(self._fiber_features(ts).float())
-                dT = self.net_f.forward(ff)[:, 0].double() * self.t_scale
-                x0[ts.idx_t] = ts.tension + dT
-                if S.nper:
-                    u, inv = self._shell_basis(ts)
+                if self.enabled_f:
+                    dT = self.net_f.forward(self.nf(ff.float()))[:, 0].double() * self.t_scale
+                    x0[ts.idx_t] += dT
+                if sh is not None and self.enabled_s:
+                    u, inv = sh
                     c = self.net_s.forward(self.ns(inv.float())).double() * self.c_scale
-                    x0[:3 * S.nper] = (u * c[:, :, None]).sum(1).reshape(-1) + ts.shell_density
-        self._pending = (self._fiber_features(ts), self._shell_basis(ts) if S.nper else None,
-                         ts.tension.clone(), ts.shell_density.clone()) if not self.trained else None
+                    x0[:3 * S.nper] += (u * c[:, :, None]).sum(1).reshape(-1)
         return x0
 
     def observe(self, ts, sol):
         S = ts.sys
-        self.prev = (self.prev + [sol.clone()])[-2:]
-        if self.trained or getattr(self, "_pending", None) is None:
+        pend = getattr(self, "_pending", None)
+        self.prev = (self.prev + [sol.clone()])[-3:]
+        if self.trained or pend is None:
             return
-        ff, sb, T_prev, q_prev = self._pending
-        T_new = sol[ts.idx_t]
-        self.samples_f.append((ff, (T_new - T_prev)[:, None]))
-        if sb is not None:
-            u, inv = sb
-            dq = sol[:3 * S.nper].view(S.nper, 3) - q_prev.view(S.nper, 3)
-            # least-squares coefficients of dq in the basis u (3 vectors)
-            G = torch.einsum("nai,nbi->nab", u, u) + 1e-12 * torch.eye(3, device=u.device)
+        ff, sh, x_ext = pend
+        dT = (sol - x_ext)[ts.idx_t]
+        self.samples_f.append((ff, dT[:, None]))
+        if sh is not None:
+            u, inv = sh
+            dq = (sol - x_ext)[:3 * S.nper].view(S.nper, 3)
+            # least-squares coefficients of the correction in the basis u
+            G = torch.einsum("nai,nbi->nab", u, u)
+            G = G + 1e-12 * (G.diagonal(dim1=1, dim2=2).sum(-1) + 1e-300)[:, None, None] \
+                * torch.eye(3, device=u.device, dtype=u.dtype)
             rhs = torch.einsum("nai,ni->na", u, dq)
             c = torch.linalg.solve(G, rhs[..., None])[..., 0]
             self.samples_s.append((inv, c))
         if len(self.samples_f) >= self.train_steps:
             self.train()
 
+    # ---- training -----------------------------------------------------------
+    def _split(self, samples):
+        nval = max(1, int(round(self.val_frac * len(samples))))
+        tr, va = samples[:-nval], samples[-nval:]
+        cat = lambda ss, i: torch.cat([s[i] for s in ss]).float()
+        return cat(tr, 0), cat(tr, 1), cat(va, 0), cat(va, 1)
+
+    def _train_one(self, samples, n_out, seed):
+        Xt, Yt, Xv, Yv = self._split(samples)
+        norm = Normalizer()
+        norm.fit(Xt)
+        scale = float(Yt.abs().max().clamp_min(1e-30))
+        net = MLP(Xt.shape[1], n_out, seed=seed)
+        net.W[2].zero_()
+        self._fit(net, norm(Xt), (Yt / scale).contiguous())
+        with torch.no_grad():
+            pred = net.forward(norm(Xv)) * scale
+            mse_net = float(((pred - Yv) ** 2).mean())
+            mse_base = float((Yv**2).mean())
+        return net, norm, scale, mse_net, mse_base
+
     def train(self):
         torch.manual_seed(self.seed)
-        Xf = torch.cat([a for a, _ in self.samples_f]).float()
-        Yf = torch.cat([b for _, b in self.samples_f]).float()
-        self.nf = Normalizer()
-        self.nf.fit(Xf)
-        self.t_scale = float(Yf.abs().max().clamp_min(1e-30))
-        self.net_f = MLP(Xf.shape[1], 1, seed=self.seed)
-        self.losses = self._fit(self.net_f, self.nf(Xf), (Yf / self.t_scale).contiguous())
+        self.net_f, self.nf, self.t_scale, mn, mb = self._train_one(self.samples_f, 1, self.seed)
+        self.enabled_f = mn < self.gate * mb
+        self.report["fiber"] = {"val_mse_net": mn, "val_mse_extrapolation": mb,
+                                "enabled": self.enabled_f}
         if self.samples_s:
-            Xs = torch.cat([a for a, _ in self.samples_s]).float()
-            Ys = torch.cat([b for _, b in self.samples_s]).float()
-            self.ns = Normalizer()
-            self.ns.fit(Xs)
-            self.c_scale = float(Ys.abs().max().clamp_min(1e-30))
-            self.net_s = MLP(Xs.shape[1], 3, seed=self.seed + 1)
-            self._fit(self.net_s, self.ns(Xs), (Ys / self.c_scale).contiguous())
+            self.net_s, self.ns, self.c_scale, mn, mb = self._train_one(self.samples_s, 3,
+                                                                        self.seed + 1)
+            self.enabled_s = mn < self.gate * mb
+            self.report["shell"] = {"val_mse_net(coeff)": mn, "val_mse_extrapolation(coeff)": mb,
+                                    "enabled": self.enabled_s}
         self.trained = True
         self.samples_f, self.samples_s = [], []
 

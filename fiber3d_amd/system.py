@@ -260,6 +260,7 @@ class FiberSystem:
         pp = self.pair_precision_rhs = "f64"
         vff = pairsum("oseen", self.X, force, self.X, src_w=self.weights, excl=self.excl,
                       eta=self.eta, reg=self.reg, eps=self.eps, precision=pp)
+        self.v_explicit = vff  # explicit flow on fiber nodes (feature of the learned guess)
         rhs = torch.zeros(self.size, dtype=torch.float64, device=dev)
         if self.shell is not None:
             vfs = pairsum("oseen", self.X, force, self.shell.pos, src_w=self.weights,

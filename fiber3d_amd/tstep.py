@@ -46,7 +46,11 @@ class TimeStepper:
                                penalty=penalty, nfd=nfd, end_force=end_force,
                                discretisation=discretisation, lengths=lengths,
                                pair_precision="f32c" if fast else "f64",
-                               lu_precision="f32c" if fast else "f64")
+                               lu_precision="f32c" if fast else "f64",
+                               # a preconditioner need not be exact: the FP32
+                               # factors are applied without refinement (a fixed
+                               # linear map), the FP64 residual judges convergence
+                               lu_refine=0)
         self.predictor = predictor
         S = self.sys
         dev = S.dev

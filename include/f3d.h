@@ -54,6 +54,16 @@ const char* f3d_last_error(void);
 /* ABI version (major*100 + minor). */
 int f3d_abi_version(void);
 
+/*
+ * Launch shape of f3d_pairsum for one (kernel, precision): k targets per lane
+ * (1, 2 or 4) and the number of wavefronts the source split aims for
+ * (64 .. 2^20); 0 restores the default (2, 8192).  Process-wide, set by the
+ * autotuner (fiber3d_amd/autotune.py); the workspace size depends on it, so
+ * query f3d_pairsum_workspace_size after changing it.  No reference
+ * counterpart (the numba kernels have no launch shape).
+ */
+int f3d_pairsum_set_launch(int kernel, int precision, int32_t k, int32_t target_waves);
+
 /* Bytes of workspace f3d_pairsum needs for these sizes. */
 size_t f3d_pairsum_workspace_size(int kernel, int precision, int64_t ns, int64_t nt);
 
