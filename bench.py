@@ -53,7 +53,7 @@ def cpu_baseline(x, f, n_targets=4096, cores=None):
             "sample": f"Oseen FP64, all {N_PTS} sources x {n_targets} targets, median of 3"}
 
 
-def system_leg(cfg_id, steps, train_steps=0, scale=None):
+def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto"):
     """time-steps/s of the implicit fiber step, plain FP64 vs fast (+ learned guess).
 
     Each mode runs its own trajectory from the same seeded initial state:
@@ -75,17 +75,22 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None):
         nper = scale[1]
     fibers = configs.make_fibers(spec, cfg.seed)
     xs, ns, ws = configs.make_periphery(nper, cfg.periphery_radius)
-    shell = Shell(xs, ns, ws, eta=cfg.eta)
+    shell = Shell(xs, ns, ws, eta=cfg.eta, pc=shell_pc, pc_f32=True)
     out = {"workload": f"{cfg.name}: {len(fibers)} fibers x {spec.num_points} nodes, "
-                       f"{nper}-node periphery, dt={cfg.dt}, GMRES tol {cfg.gmres_tol}"}
+                       f"{nper}-node periphery, dt={cfg.dt}, GMRES tol {cfg.gmres_tol}"
+                       + (" (rigid bodies not included)" if cfg_id == 4 else ""),
+           "shell_preconditioner": shell.pc}
     modes = [("plain", "plain", None), ("fast", "fast", None)]
     if train_steps:
-        modes.append(("learned_fast", "fast", LearnedGuess(train_steps=train_steps)))
+        modes.append(("extrapolated_fast", "fast", LearnedGuess(train_steps=10**9, use_net=False)))
+        modes.append(("learned_fast", "fast", LearnedGuess(train_steps=train_steps, epochs=40)))
     for name, mode, pred in modes:
         ts = TimeStepper(fibers, shell, mode=mode, eta=cfg.eta, dt=cfg.dt, E=spec.E,
                          epsilon=spec.epsilon, tol=cfg.gmres_tol, restart=cfg.restart,
                          maxiter=cfg.maxiter, predictor=pred, lengths=np.full(len(fibers), spec.length))
-        warm = max(1, train_steps if pred is not None else 1)
+        # the learned mode collects samples from its 4th step on and trains
+        # after train_steps of them: all of that happens before timing starts
+        warm = train_steps + 4 if pred is not None else 1
         for _ in range(warm):
             ts.step()
         torch.cuda.synchronize()
@@ -99,11 +104,14 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None):
         dt = time.perf_counter() - t0
         out[name] = {"time_steps_per_s": round(steps / dt, 4), "mean_gmres_iters": float(np.mean(its)),
                      "max_true_residual": float(np.max(res)), "steps": steps, "untimed_steps": warm}
+        if pred is not None and pred.use_net:
+            out[name]["predictor"] = pred.report
     if "plain" in out and "fast" in out:
         out["speedup_fast_vs_plain"] = round(out["fast"]["time_steps_per_s"] / out["plain"]["time_steps_per_s"], 3)
-    if "learned_fast" in out:
-        out["speedup_learned_fast_vs_plain"] = round(
-            out["learned_fast"]["time_steps_per_s"] / out["plain"]["time_steps_per_s"], 3)
+    for m in ("extrapolated_fast", "learned_fast"):
+        if m in out:
+            out[f"speedup_{m}_vs_plain"] = round(
+                out[m]["time_steps_per_s"] / out["plain"]["time_steps_per_s"], 3)
     return out
 
 
@@ -172,6 +180,7 @@ def main():
     ap.add_argument("--system-steps", type=int, default=5)
     ap.add_argument("--train-steps", type=int, default=0)
     ap.add_argument("--lu", action="store_true", help="also time the batched fiber LU (workload 2)")
+    ap.add_argument("--shell-pc", default="auto", choices=["auto", "dense", "block"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
@@ -257,7 +266,8 @@ def main():
     if args.lu:
         line["batched_lu"] = lu_leg(cpu=not args.no_cpu)
     if args.system >= 0:
-        line["fiber_system"] = system_leg(args.system, args.system_steps, args.train_steps)
+        line["fiber_system"] = system_leg(args.system, args.system_steps, args.train_steps,
+                                          shell_pc=args.shell_pc)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(x, f)
     print(json.dumps(line), flush=True)

@@ -77,4 +77,5 @@ def tune_pairsum(n=65536, kernels=("oseen", "stresslet", "rotlet"), precisions=(
 
 
 if __name__ == "__main__":
-    print(json.dumps(tune_pairsum(), indent=1))
+    import sys
+    print(json.dumps(tune_pairsum(path=sys.argv[1] if len(sys.argv) > 1 else CACHE), indent=1))

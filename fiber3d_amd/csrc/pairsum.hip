@@ -16,7 +16,7 @@
 //     no LDS traffic and no VGPRs spent on sources.  Sources are split into
 //     P chunks over grid.y so the launch carries thousands of waves.
 //   * FP32C: positions are quantised once per call to 31-bit integers on the
-//     bounding box of sources and targets (bbox_kernel), so d = t - s is an
+//     bounding box of sources and targets (bbox_partial/final), so d = t - s is an
 //     exact integer difference rounded once to FP32; the pair is then
 //     evaluated in FP32 in units of the quantum h and the sum rescaled by
 //     h^-p (the kernels are homogeneous of degree -1 / -2 in d).  Rounding the
@@ -76,23 +76,11 @@ struct Geo {
 };
 
 // ---------------------------------------------------------------------------
-// Bounding box of sources and targets -> Geo (one workgroup).
-__global__ void __launch_bounds__(1024)
-bbox_kernel(int ns, const double* __restrict__ pos, long sp, long sc, int nt,
-            const double* __restrict__ tpos, long t_sp, long t_sc, double min_extent,
-            Geo* __restrict__ geo) {
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int j = threadIdx.x; j < ns + nt; j += blockDim.x) {
-    const double* p = j < ns ? pos + (long)j * sp : tpos + (long)(j - ns) * t_sp;
-    const long c = j < ns ? sc : t_sc;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double v = p[a * c];
-      lo[a] = fmin(lo[a], v);
-      hi[a] = fmax(hi[a], v);
-    }
-  }
-  __shared__ double red[16][6];
+// Bounding box of sources and targets -> Geo, in two passes: per-block
+// min/max (kBoxBlocks blocks, grid-stride), then one block combines them.
+constexpr int kBoxBlocks = 128;
+
+__device__ __forceinline__ void block_minmax(double (&lo)[3], double (&hi)[3], double* red) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
 #pragma unroll
@@ -105,22 +93,64 @@ bbox_kernel(int ns, const double* __restrict__ pos, long sp, long sc, int nt,
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      red[w][a] = lo[a];
-      red[w][3 + a] = hi[a];
+      red[w * 6 + a] = lo[a];
+      red[w * 6 + 3 + a] = hi[a];
     }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int nw = blockDim.x >> 6;
-    double ext = 0.0;
-    for (int a = 0; a < 3; ++a) {
-      double l = red[0][a], u = red[0][3 + a];
-      for (int k = 1; k < nw; ++k) {
-        l = fmin(l, red[k][a]);
-        u = fmax(u, red[k][3 + a]);
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = fmin(lo[a], red[k * 6 + a]);
+        hi[a] = fmax(hi[a], red[k * 6 + 3 + a]);
       }
-      geo->origin[a] = l;
-      ext = fmax(ext, u - l);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+bbox_partial_kernel(int ns, const double* __restrict__ pos, long sp, long sc, int nt,
+                    const double* __restrict__ tpos, long t_sp, long t_sc,
+                    double* __restrict__ part) {
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ns + nt; j += gridDim.x * blockDim.x) {
+    const double* p = j < ns ? pos + (long)j * sp : tpos + (long)(j - ns) * t_sp;
+    const long c = j < ns ? sc : t_sc;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double v = p[a * c];
+      lo[a] = fmin(lo[a], v);
+      hi[a] = fmax(hi[a], v);
+    }
+  }
+  __shared__ double red[4 * 6];
+  block_minmax(lo, hi, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      part[blockIdx.x * 6 + a] = lo[a];
+      part[blockIdx.x * 6 + 3 + a] = hi[a];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBoxBlocks)
+bbox_final_kernel(const double* __restrict__ part, double min_extent, Geo* __restrict__ geo) {
+  double lo[3], hi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = part[threadIdx.x * 6 + a];
+    hi[a] = part[threadIdx.x * 6 + 3 + a];
+  }
+  __shared__ double red[(kBoxBlocks / 64) * 6];
+  block_minmax(lo, hi, red);
+  if (threadIdx.x == 0) {
+    double ext = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      geo->origin[a] = lo[a];
+      ext = fmax(ext, hi[a] - lo[a]);
     }
     ext = fmax(ext, min_extent);
     geo->h = ext / kQuantLevels;
@@ -497,7 +527,8 @@ static size_t part_bytes(int precision, int P, int64_t nt) {
 static size_t pairsum_ws(int kernel, int precision, int64_t ns, int64_t nt) {
   const Plan pl = make_plan(kernel, precision, ns, nt);
   return rec_bytes(kernel, precision, pl.ns_pad) + part_bytes(precision, pl.P, nt) +
-         align_up(kPackBlocks * sizeof(double), 256) + align_up(sizeof(Geo), 256);
+         align_up(kPackBlocks * sizeof(double), 256) + align_up(sizeof(Geo), 256) +
+         align_up(kBoxBlocks * 6 * sizeof(double), 256);
 }
 
 template <typename T, int KIND>
@@ -517,12 +548,15 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
   double* sigma_part = reinterpret_cast<double*>(p);
   p += align_up(kPackBlocks * sizeof(double), 256);
   Geo* geo = kQuant ? reinterpret_cast<Geo*>(p) : nullptr;
+  double* box_part = reinterpret_cast<double*>(p + align_up(sizeof(Geo), 256));
 
   if (kQuant) {
     // the quantum must resolve eps; an extent below 64 (eps + reg) is raised
     const double min_extent = 64.0 * (eps + reg) + 1e-300;
-    F3D_LAUNCH(bbox_kernel, dim3(1), dim3(1024), 0, st, (int)ns, pos, (long)s_sp, (long)s_sc,
-               (int)nt, tpos, (long)t_sp, (long)t_sc, min_extent, geo);
+    F3D_LAUNCH(bbox_partial_kernel, dim3(kBoxBlocks), dim3(256), 0, st, (int)ns, pos, (long)s_sp,
+               (long)s_sc, (int)nt, tpos, (long)t_sp, (long)t_sc, box_part);
+    F3D_LAUNCH(bbox_final_kernel, dim3(1), dim3(kBoxBlocks), 0, st, (const double*)box_part,
+               min_extent, geo);
   }
   const bool want_sigma = epi_in != nullptr;
   F3D_LAUNCH((pack_kernel<T, KIND>), dim3(kPackBlocks), dim3(256), 0, st, (int)ns, pl.ns_pad,

@@ -44,10 +44,20 @@ def _c0c1(epsilon, eta):
 
 
 class Shell:
-    """Static periphery: nodes, inward normals, weights, subtraction vectors,
-    dense preconditioner (built and LU-factored once, as tstep.py:278-356)."""
+    """Static periphery: nodes, inward normals, weights, subtraction vectors and
+    the shell block of the preconditioner.
 
-    def __init__(self, x, normals, weights, eta=1.0, reg=5e-3, eps=1e-5, dense_pc=True):
+    pc='dense' builds the reference's dense operator (tstep.py:328-345) and
+    LU-factors it once (FP64 as in the reference; FP32 as well when
+    pc_f32=True, for the fast mode).  pc='block' inverts only each node's 3x3
+    diagonal block, -E_i/w_i - I/w_i + n_i n_i^T: the dense factor of a
+    32768-node shell would be 98304^2 (77 GB in FP64), so large shells use
+    this instead.  pc='auto' picks 'dense' up to 12000 nodes."""
+
+    DENSE_MAX = 12000
+
+    def __init__(self, x, normals, weights, eta=1.0, reg=5e-3, eps=1e-5, pc="auto",
+                 pc_f32=False):
         dev = device()
         self.n = x.shape[0]
         self.pos = torch.from_numpy(np.ascontiguousarray(x, np.float64)).to(dev)
@@ -61,9 +71,24 @@ class Shell:
             dens[:, c] = self.w
             self.e.append(pairsum("stresslet", self.pos, dens, self.pos, src_nrm=self.nrm,
                                   eta=eta, reg=reg, eps=eps))
-        self.lu = None
-        if dense_pc:
-            self.build_dense_pc()
+        if pc == "auto":
+            pc = "dense" if self.n <= self.DENSE_MAX else "block"
+        self.pc = pc
+        self.lu = self.lu32 = None
+        if pc == "dense":
+            M = self.dense_matrix()
+            if pc_f32:
+                self.lu32, self.piv32 = torch.linalg.lu_factor(M.float())
+            self.lu, self.piv = torch.linalg.lu_factor(M)
+            del M
+        elif pc == "block":
+            E = torch.stack(self.e, dim=2)  # E[i, a, c] = e_c[i, a]
+            winv = (1.0 / self.w)[:, None, None]
+            I3 = torch.eye(3, dtype=torch.float64, device=dev)[None]
+            D = -E * winv - I3 * winv + self.nrm[:, :, None] * self.nrm[:, None, :]
+            self.Dinv = torch.linalg.inv(D)
+        else:
+            raise ValueError("pc is 'auto', 'dense' or 'block'")
 
     def dense_matrix(self):
         """M of tstep.py:328-345 on the GPU (3N x 3N, AoS ordering)."""
@@ -84,18 +109,19 @@ class Shell:
         M += torch.outer(nflat, nflat)
         return M
 
-    def build_dense_pc(self):
-        M = self.dense_matrix()
-        self.lu, self.piv = torch.linalg.lu_factor(M)
-        del M
-
     def apply(self, q, add=None, out=None, precision="f64"):
         return periphery_apply(self.pos, self.nrm, self.w, self.e[0], self.e[1], self.e[2], q,
                                add=add, eta=self.eta, reg=self.reg, eps=self.eps,
                                precision=precision, out=out)
 
-    def precond(self, r, out):
-        out.copy_(torch.linalg.lu_solve(self.lu, self.piv, r.reshape(-1, 1)).reshape(-1))
+    def precond(self, r, out, f32=False):
+        if self.pc == "block":
+            out.view(self.n, 3).copy_(torch.matmul(self.Dinv, r.view(self.n, 3, 1))[..., 0])
+        elif f32 and self.lu32 is not None:
+            out.copy_(torch.linalg.lu_solve(self.lu32, self.piv32, r.float().reshape(-1, 1))
+                      .reshape(-1))
+        else:
+            out.copy_(torch.linalg.lu_solve(self.lu, self.piv, r.reshape(-1, 1)).reshape(-1))
         return out
 
 
@@ -321,7 +347,7 @@ class FiberSystem:
             out = torch.empty_like(x)
         nper3 = 3 * self.nper
         if self.shell is not None:
-            self.shell.precond(x[:nper3], out[:nper3])
+            self.shell.precond(x[:nper3], out[:nper3], f32=self.lu_precision == "f32c")
         self.lu.solve(x[nper3:], out=out[nper3:])
         return out
 

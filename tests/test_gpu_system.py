@@ -87,3 +87,19 @@ def test_config0_gmres_to_1e8():
     torch.cuda.synchronize()
     assert res32.info == 0 and res32.residual <= 1e-8
     assert rel2(res32.x.cpu().numpy(), x) < 1e-6
+
+
+def test_block_jacobi_shell_preconditioner_same_solution():
+    """The 3x3 block-Jacobi shell preconditioner (used for large shells) changes
+    only the iteration count, not the solution."""
+    print("case shell block pc", flush=True)
+    fibers = configs.make_fibers(configs.FiberSpec(8, 16, 1.0, 2.0), seed=5)
+    xs, ns, ws = configs.make_periphery(300, 3.5)
+    dense = FiberSystem(fibers, Shell(xs, ns, ws, pc="dense"))
+    block = FiberSystem(fibers, Shell(xs, ns, ws, pc="block"))
+    r1 = gmres(dense.matvec, dense.rhs, dense.precond, tol=1e-10)
+    r2 = gmres(block.matvec, block.rhs, block.precond, tol=1e-10)
+    torch.cuda.synchronize()
+    print("iterations dense / block:", r1.iterations, r2.iterations)
+    assert r1.info == 0 and r2.info == 0
+    assert rel2(r2.x.cpu().numpy(), r1.x.cpu().numpy()) < 1e-7
