@@ -116,3 +116,33 @@ def make_periphery(n, radius):
     x, nout = shape_gallery_sphere(n, radius)
     w = np.full(n, 4.0 * np.pi * radius * radius / n)
     return x, -nout, w
+
+
+def make_sphere_body(centre, radius=0.5, nodes=600):
+    """Rigid sphere: Fibonacci nodes (shape_gallery.py:7-32), outward normals,
+    equal quadrature weights 4 pi a^2 / N."""
+    x, n = shape_gallery_sphere(nodes, radius)
+    return (np.asarray(centre, float), x + np.asarray(centre, float)[None, :], n,
+            np.full(nodes, 4.0 * np.pi * radius * radius / nodes))
+
+
+def make_bodies(count, radius, nodes, ball_radius, fibers=(), margin=0.1, seed=0):
+    """Spheres with centres uniform in a ball, no overlap with each other or with
+    fiber nodes (clearance `margin`)."""
+    rng = np.random.default_rng(seed)
+    fib = np.concatenate(fibers) if len(fibers) else np.zeros((0, 3))
+    tree = cKDTree(fib) if len(fib) else None
+    centres = []
+    for _ in range(10000 * max(count, 1)):
+        if len(centres) == count:
+            break
+        h = rng.normal(size=3)
+        c = ball_radius * rng.uniform() ** (1.0 / 3.0) * h / np.linalg.norm(h)
+        if any(np.linalg.norm(c - d) < 2 * radius + margin for d in centres):
+            continue
+        if tree is not None and tree.query(c)[0] < radius + margin:
+            continue
+        centres.append(c)
+    if len(centres) < count:
+        raise RuntimeError("body placement failed")
+    return [make_sphere_body(c, radius, nodes) for c in centres]

@@ -13,8 +13,13 @@ force/torque-free ends inside a spherical periphery:
   * the block preconditioner P_inv (tstep.py:950-990): batched fiber LU solves
     and a dense shell LU;
   * right-preconditioned FP64 GMRES (gmres.py in this package).
+  * free rigid bodies (no fiber attached): surface double layer with
+    singularity subtraction, K = [J, rot], the body rows of the matvec
+    (tstep.py:1044-1143), external force/torque as a Stokeslet + rotlet at the
+    centre (tstep.py:818-846) and a dense per-body preconditioner
+    (tstep_utils.py:1341-1387).
 Unknown vector layout is the reference's: [shell density (3 Nper, AoS) |
-per fiber X(N), Y(N), Z(N), T(N)].
+per body density (3 Nb, AoS), U, Omega | per fiber X(N), Y(N), Z(N), T(N)].
 """
 import ctypes
 
@@ -125,11 +130,147 @@ class Shell:
         return out
 
 
+class Bodies:
+    """Free rigid bodies on the GPU: surface nodes, outward normals, weights,
+    singularity-subtraction vectors, K = [J, rot] (body/body.py:162-193,
+    :325-346) and the dense per-body preconditioner block
+    [M - I_sub, -K; -K^T, I] (tstep_utils.py:1341-1387), LU-factored once per
+    step.  All bodies must carry the same node count (batched factor)."""
+
+    def __init__(self, bodies, eta, reg, eps, body_force=None):
+        dev = device()
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+        sizes = [np.asarray(b[1]).shape[0] for b in bodies]
+        if len(set(sizes)) != 1:
+            raise ValueError("bodies must have equal node counts")
+        self.count = len(bodies)
+        self.nb = sizes[0]
+        self.n = self.count * self.nb
+        self.eta, self.reg, self.eps = eta, reg, eps
+        self.centres = t(np.array([b[0] for b in bodies]))
+        self.pos = t(np.concatenate([b[1] for b in bodies]))
+        self.nrm = t(np.concatenate([b[2] for b in bodies]))
+        self.w = t(np.concatenate([b[3] for b in bodies]))
+        self.bid = torch.arange(self.count, device=dev).repeat_interleave(self.nb)
+        f = np.zeros((self.count, 6)) if body_force is None else np.asarray(body_force, float)
+        self.force = t(f)
+        self.vb = torch.zeros((self.n, 3), dtype=torch.float64, device=dev)
+        self.update_geometry()
+
+    def update_geometry(self):
+        """Subtraction vectors and the preconditioner for the current pose."""
+        nb = self.nb
+        self.r = self.pos - self.centres[self.bid]
+        self.e = []
+        for c in range(3):
+            dens = torch.zeros_like(self.pos)
+            dens[:, c] = self.w
+            e = torch.empty_like(self.pos)
+            for k in range(self.count):
+                s = slice(k * nb, (k + 1) * nb)
+                pairsum("stresslet", self.pos[s], dens[s], self.pos[s], src_nrm=self.nrm[s],
+                        eta=self.eta, reg=self.reg, eps=self.eps, out=e[s])
+            self.e.append(e)
+        m = 3 * nb + 6
+        A = torch.zeros((self.count, m, m), dtype=torch.float64, device=self.pos.device)
+        for k in range(self.count):
+            s = slice(k * nb, (k + 1) * nb)
+            Mk = A[k, :3 * nb, :3 * nb]
+            rc = L.load().f3d_stresslet_matrix(nb, L.ptr(self.pos[s]), L.ptr(self.nrm[s]), 3, 1,
+                                               float(self.eta), float(self.reg), float(self.eps),
+                                               L.ptr(A[k]), m, _stream())
+            L.check(rc, "f3d_stresslet_matrix")
+            Mv = Mk.view(nb, 3, nb, 3)
+            idx = torch.arange(nb, device=A.device)
+            winv = 1.0 / self.w[s]
+            for c in range(3):
+                Mv[idx, :, idx, c] -= self.e[c][s] * winv[:, None]
+            Kb = self.k_matrix(self.r[s])
+            A[k, :3 * nb, 3 * nb:] = -Kb
+            A[k, 3 * nb:, :3 * nb] = -Kb.t()
+            A[k, 3 * nb:, 3 * nb:] = torch.eye(6, dtype=torch.float64, device=A.device)
+        self.lu, self.piv = torch.linalg.lu_factor(A)
+
+    @staticmethod
+    def k_matrix(r):
+        n = r.shape[0]
+        K = torch.zeros((n, 3, 6), dtype=torch.float64, device=r.device)
+        K[:, 0, 0] = K[:, 1, 1] = K[:, 2, 2] = 1.0
+        x, y, z = r[:, 0], r[:, 1], r[:, 2]
+        K[:, 0, 4], K[:, 0, 5] = z, -y
+        K[:, 1, 3], K[:, 1, 5] = -z, x
+        K[:, 2, 3], K[:, 2, 4] = y, -x
+        return K.reshape(3 * n, 6)
+
+    def block(self, vec, start):
+        """(count, 3 Nb + 6) view of the body unknowns: per body the density
+        (AoS) followed by U and Omega, as in the reference's layout
+        (tstep.py:1017-1025)."""
+        m = 3 * self.nb + 6
+        return vec[start:start + self.count * m].view(self.count, m)
+
+    def density_view(self, x, start):
+        """Contiguous (n, 3) copy of all body densities."""
+        return self.block(x, start)[:, :3 * self.nb].reshape(self.n, 3)
+
+    def _vel(self, x, start):
+        return self.block(x, start)[:, 3 * self.nb:]
+
+    def explicit(self, trg, out, accumulate=True):
+        """Stokeslet + rotlet of the external force/torque at each centre."""
+        kw = dict(eta=self.eta, reg=self.reg, eps=self.eps)
+        pairsum("oseen", self.centres, self.force[:, :3].contiguous(), trg, out=out,
+                accumulate=accumulate, **kw)
+        pairsum("rotlet", self.centres, self.force[:, 3:].contiguous(), trg, out=out,
+                accumulate=True, **kw)
+        return out
+
+    def scatter_density(self, vec, v, start):
+        """Write per-node 3-vectors v (n, 3) into the density slots of vec."""
+        self.block(vec, start)[:, :3 * self.nb] = v.reshape(self.count, 3 * self.nb)
+
+    def rows(self, x, out, start):
+        """Body density rows vb - sum_c (lam_c / w) e_c - K U and velocity rows
+        -K^T lam + U (tstep.py:1121-1143)."""
+        lam = self.density_view(x, start)
+        vel = self._vel(x, start)
+        a = lam / self.w[:, None]
+        sub = a[:, 0:1] * self.e[0] + a[:, 1:2] * self.e[1] + a[:, 2:3] * self.e[2]
+        U = vel[self.bid, :3]
+        W = vel[self.bid, 3:]
+        KU = U + torch.cross(W, self.r, dim=1)
+        ob = self.block(out, start)
+        ob[:, :3 * self.nb] = (self.vb - sub - KU).reshape(self.count, 3 * self.nb)
+        F = torch.zeros((self.count, 3), dtype=torch.float64, device=x.device).index_add_(0, self.bid, lam)
+        T = torch.zeros_like(F).index_add_(0, self.bid, torch.cross(self.r, lam, dim=1))
+        ob[:, 3 * self.nb:3 * self.nb + 3] = vel[:, :3] - F
+        ob[:, 3 * self.nb + 3:] = vel[:, 3:] - T
+
+    def precond(self, x, out, start):
+        xb = self.block(x, start)
+        self.block(out, start).copy_(torch.linalg.lu_solve(self.lu, self.piv, xb[..., None])[..., 0])
+
+    def move(self, vel, dt):
+        """Rigid update: centre += U dt, nodes rotated by Omega dt (Rodrigues)."""
+        U, W = vel[:, :3], vel[:, 3:]
+        th = torch.linalg.norm(W, dim=1) * dt
+        ax = W / torch.linalg.norm(W, dim=1, keepdim=True).clamp_min(1e-300)
+        c, s = torch.cos(th)[:, None], torch.sin(th)[:, None]
+        def rot(v):
+            k = ax[self.bid]
+            cc, ss = c[self.bid], s[self.bid]
+            return v * cc + torch.cross(k, v, dim=1) * ss + k * (k * v).sum(1, keepdim=True) * (1 - cc)
+        self.centres = self.centres + U * dt
+        self.pos = self.centres[self.bid] + rot(self.r)
+        self.nrm = rot(self.nrm)
+        self.update_geometry()
+
+
 class FiberSystem:
     def __init__(self, fibers, shell=None, eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
                  lengths=None, pair_precision="f64", lu_precision="f64", lu_refine=6,
-                 lu_tol=1e-13, reg=5e-3, eps=1e-5):
+                 lu_tol=1e-13, reg=5e-3, eps=1e-5, bodies=None, body_force=None):
         self.dev = device()
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = _c0c1(epsilon, eta)
@@ -147,7 +288,11 @@ class FiberSystem:
         self.npts = int(self.offs[-1])
         self.shell = shell
         self.nper = 0 if shell is None else shell.n
-        self.size = 3 * self.nper + 4 * self.npts
+        self.bodies = Bodies(bodies, eta, reg, eps, body_force) if bodies else None
+        self.nbody = 0 if self.bodies is None else self.bodies.count
+        self.nbn = 0 if self.bodies is None else self.bodies.n
+        self.fib0 = 3 * self.nper + 3 * self.nbn + 6 * self.nbody
+        self.size = self.fib0 + 4 * self.npts
         self.lengths_h = np.full(self.nfib, 1.0) if lengths is None else np.asarray(lengths, float)
         self.lengths = torch.from_numpy(self.lengths_h).to(self.dev)
         # node positions (npts, 3), AoS
@@ -288,9 +433,19 @@ class FiberSystem:
                       eta=self.eta, reg=self.reg, eps=self.eps, precision=pp)
         self.v_explicit = vff  # explicit flow on fiber nodes (feature of the learned guess)
         rhs = torch.zeros(self.size, dtype=torch.float64, device=dev)
+        B = self.bodies
+        if B is not None:
+            # external body force/torque: Stokeslet + rotlet at each centre
+            B.explicit(self.X, out=vff, accumulate=True)
+            vb = pairsum("oseen", self.X, force, B.pos, src_w=self.weights, eta=self.eta,
+                         reg=self.reg, eps=self.eps, precision=pp)
+            B.explicit(B.pos, out=vb, accumulate=True)
+            B.scatter_density(rhs, -vb, 3 * self.nper)
         if self.shell is not None:
             vfs = pairsum("oseen", self.X, force, self.shell.pos, src_w=self.weights,
                           eta=self.eta, reg=self.reg, eps=self.eps, precision=pp)
+            if B is not None:
+                B.explicit(self.shell.pos, out=vfs, accumulate=True)
             rhs[:3 * self.nper] = -vfs.reshape(-1)
         dt, c0, c1, p = self.dt, self.c0, self.c1, self.penalty
         for N, (node, X, xs, xss, D1, P, B) in self._ctx.items():
@@ -310,32 +465,46 @@ class FiberSystem:
             Fend = fk[:, -1]
             Rsq[:, 4 * N - 7:4 * N - 4] = Fend
             Rsq[:, 4 * N - 4] = (Fend * xs[:, -1]).sum(-1)
-            _scatter_blocks(rhs, 3 * self.nper + 4 * self.offs[bk["ids"]], Rsq)
+            _scatter_blocks(rhs, self.fib0 + 4 * self.offs[bk["ids"]], Rsq)
         self.rhs = rhs
 
     # ------------------------------------------------------------------
     def matvec(self, x, out=None, precision=None):
-        """A_body_fiber_hyd (tstep.py:1003-1178) for fibers + shell."""
+        """A_body_fiber_hyd (tstep.py:1003-1178) for fibers, free bodies and shell."""
         prec = precision or self.pair_precision
         if out is None:
             out = torch.empty_like(x)
         nper3 = 3 * self.nper
-        XT = x[nper3:]
+        XT = x[self.fib0:]
+        B = self.bodies
+        kw = dict(eta=self.eta, reg=self.reg, eps=self.eps, precision=prec)
         # fiber forces f = F XT into SoA (3, npts)
         self.gF(self.F_buf, XT, self.f_soa)
         fsoa = self._soa(self.f_soa)
         vsoa = self._soa(self.v_soa)
-        pairsum("oseen", self.X, fsoa, self.X, src_w=self.weights, excl=self.excl, eta=self.eta,
-                reg=self.reg, eps=self.eps, precision=prec, out=vsoa)
+        pairsum("oseen", self.X, fsoa, self.X, src_w=self.weights, excl=self.excl, out=vsoa, **kw)
+        if B is not None:
+            lam = B.density_view(x, nper3)
+            vb = B.vb
+            # body double layer on fibers and bodies (the j = i term is exactly 0)
+            pairsum("stresslet", B.pos, lam, self.X, src_nrm=B.nrm, out=vsoa, accumulate=True, **kw)
+            pairsum("stresslet", B.pos, lam, B.pos, src_nrm=B.nrm, out=vb, **kw)
+            pairsum("oseen", self.X, fsoa, B.pos, src_w=self.weights, out=vb, accumulate=True, **kw)
         if self.shell is not None:
             q = x[:nper3].view(self.nper, 3)
-            pairsum("oseen", self.X, fsoa, self.shell.pos, src_w=self.weights, eta=self.eta,
-                    reg=self.reg, eps=self.eps, precision=prec, out=self.v_shell)
+            pairsum("oseen", self.X, fsoa, self.shell.pos, src_w=self.weights, out=self.v_shell, **kw)
+            if B is not None:
+                pairsum("stresslet", B.pos, lam, self.shell.pos, src_nrm=B.nrm, out=self.v_shell,
+                        accumulate=True, **kw)
+                pairsum("stresslet", self.shell.pos, q, B.pos, src_nrm=self.shell.nrm, out=vb,
+                        accumulate=True, **kw)
             self.shell.apply(q, add=self.v_shell, out=out[:nper3].view(self.nper, 3),
                              precision=prec)
-            pairsum("stresslet", self.shell.pos, q, self.X, src_nrm=self.shell.nrm, eta=self.eta,
-                    reg=self.reg, eps=self.eps, precision=prec, out=vsoa, accumulate=True)
-        res = out[nper3:]
+            pairsum("stresslet", self.shell.pos, q, self.X, src_nrm=self.shell.nrm, out=vsoa,
+                    accumulate=True, **kw)
+        if B is not None:
+            B.rows(x, out, nper3)
+        res = out[self.fib0:]
         self.gA(self.A_buf, XT, res)
         for c in range(3):
             self.gQ[c](self.Q_buf[c], self.v_soa, res, alpha=-1.0, beta=1.0)
@@ -348,7 +517,9 @@ class FiberSystem:
         nper3 = 3 * self.nper
         if self.shell is not None:
             self.shell.precond(x[:nper3], out[:nper3], f32=self.lu_precision == "f32c")
-        self.lu.solve(x[nper3:], out=out[nper3:])
+        if self.bodies is not None:
+            self.bodies.precond(x, out, nper3)
+        self.lu.solve(x[self.fib0:], out=out[self.fib0:])
         return out
 
     def dense_blocks_host(self):

@@ -3,9 +3,10 @@
 One step follows tstep.time_step_hydro (python/tstep/tstep.py:628-1216):
 assemble (FiberSystem.prepare), solve with right-preconditioned GMRES to the
 relative tolerance, then update fiber positions, tensions and the shell
-density from the solution (tstep.py:1389-1397).  Adaptive step rejection,
-dynamic instability and output writing of the reference driver are not part
-of this path.
+density from the solution (tstep.py:1389-1397).  Free rigid bodies move
+rigidly with their solved velocities.  Adaptive step rejection, dynamic
+instability and output writing of the reference driver are not part of this
+path.
 
 Two modes:
   'plain'  : FP64 pair sums, FP64 fiber LU, zero initial guess (the reference
@@ -36,7 +37,8 @@ class StepStats:
 class TimeStepper:
     def __init__(self, fibers, shell, mode="plain", eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference", tol=1e-8,
-                 restart=150, maxiter=200, predictor=None, lengths=None):
+                 restart=150, maxiter=200, predictor=None, lengths=None, bodies=None,
+                 body_force=None):
         if mode not in ("plain", "fast"):
             raise ValueError("mode is 'plain' or 'fast'")
         self.mode = mode
@@ -45,6 +47,7 @@ class TimeStepper:
         self.sys = FiberSystem(fibers, shell, eta=eta, dt=dt, E=E, epsilon=epsilon,
                                penalty=penalty, nfd=nfd, end_force=end_force,
                                discretisation=discretisation, lengths=lengths,
+                               bodies=bodies, body_force=body_force,
                                pair_precision="f32c" if fast else "f64",
                                lu_precision="f32c" if fast else "f64",
                                # a preconditioner need not be exact: the FP32
@@ -59,7 +62,7 @@ class TimeStepper:
         tidx = np.empty(S.npts, dtype=np.int64)
         for k in range(S.nfib):
             N = S.sizes[k]
-            base = 3 * S.nper + 4 * S.offs[k]
+            base = S.fib0 + 4 * S.offs[k]
             for c in range(3):
                 idx[S.offs[k]:S.offs[k + 1], c] = base + c * N + np.arange(N)
             tidx[S.offs[k]:S.offs[k + 1]] = base + 3 * N + np.arange(N)
@@ -92,6 +95,9 @@ class TimeStepper:
         self.shell_density = sol[:3 * S.nper].clone()
         self.tension = sol[self.idx_t]
         S.X = sol[self.idx_x].contiguous()
+        if S.bodies is not None:
+            # rigid motion with the solved velocities (tstep.py:1398-1406)
+            S.bodies.move(S.bodies._vel(sol, 3 * S.nper), S.dt)
         S.prepare()
         torch.cuda.synchronize()
         st = StepStats(res.iterations, res.restarts, res.residual, time.perf_counter() - t0,

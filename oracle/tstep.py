@@ -1,7 +1,8 @@
 """Oracle: one implicit hydrodynamic time step of fibers in a periphery (FP64).
 
-Restates, for free fibers (force/torque-free ends, no bodies) inside a
-spherical shell, the path of tstep.time_step_hydro (python/tstep/tstep.py:628-1216):
+Restates, for free fibers (force/torque-free ends) and free rigid bodies (no
+fiber attached) inside a spherical shell, the path of tstep.time_step_hydro
+(python/tstep/tstep.py:628-1216):
   * explicit flows of the external fiber forces and the system RHS
     (tstep.py:757-846, tstep_utils.get_fibers_and_bodies_matrices
     tstep_utils.py:1238-1336, flow_fibers :653-700, self_flow_fibers :812-851);
@@ -10,11 +11,19 @@ spherical shell, the path of tstep.time_step_hydro (python/tstep/tstep.py:628-12
   * right-preconditioned restarted GMRES (bpm_utilities/gmres.py:94-108 over
     SciPy's gmres, tstep.py:1209-1216), here restated as textbook GMRES
     (Arnoldi with modified Gram-Schmidt, Givens rotations).
-The unknown vector is [shell density (3 Nper, AoS) | per fiber X, Y, Z, T].
+  * rigid bodies: surface double layer with singularity subtraction, K = [J, rot]
+    (body/body.py:162-193, :325-346), the body rows of the matvec
+    (tstep.py:1044-1143), external force/torque as a Stokeslet and rotlet at
+    the centre (tstep.py:818-846), the dense body preconditioner
+    (tstep_utils.py:1341-1387).
+The unknown vector is [shell density (3 Nper, AoS) | per body density (3 Nb,
+AoS), U, Omega | per fiber X, Y, Z, T].
 
 Pinned by tests/test_oracle_tstep.py: the matvec against an independently
-assembled dense operator (config-0-sized), the preconditioner blocks against
-P^-1 (A x) = x, GMRES against a dense direct solve.  The reference's matvec
+assembled dense operator (fibers and shell), the preconditioner blocks against
+P^-1 (A x) = x, GMRES against a dense direct solve, and the body formulation
+against Stokes' law for a lone sphere (U = F/(6 pi eta a), Omega =
+L/(8 pi eta a^3)).  The reference's matvec
 and SciPy's GMRES cannot be imported here (stkfmm / mpi4py / numba.errors
 imports), so beyond those invariants the system assembly is parity unpinned.
 """
@@ -29,8 +38,10 @@ from . import periphery as P
 class HydroSystem:
     def __init__(self, fibers, shell=None, eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
-                 lengths=None):
-        """fibers: list of (N_k,3) arrays; shell: (x, normals, weights) or None."""
+                 lengths=None, bodies=None, body_force=None):
+        """fibers: list of (N_k,3) arrays; shell: (x, normals, weights) or None;
+        bodies: list of (centre (3,), nodes (Nb,3), outward normals, weights);
+        body_force: optional (nbody, 6) external force and torque per body."""
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = OF.constants(epsilon, eta)
         self.fibers = [np.asarray(x, float) for x in fibers]
@@ -41,8 +52,15 @@ class HydroSystem:
         self.npts = int(self.offs[-1])
         self.shell = shell
         self.nper = 0 if shell is None else shell[0].shape[0]
-        self.size = 3 * self.nper + 4 * self.npts
-        self.trg_fib = np.concatenate(self.fibers)
+        self.bodies = [] if bodies is None else [tuple(np.asarray(a, float) for a in b) for b in bodies]
+        self.nbody = len(self.bodies)
+        self.bsizes = np.array([b[1].shape[0] for b in self.bodies], dtype=int)
+        self.boffs = np.concatenate([[0], np.cumsum(self.bsizes)]).astype(int)
+        self.body_force = np.zeros((self.nbody, 6)) if body_force is None else np.asarray(body_force, float)
+        # unknowns: shell | per body (3 Nb density, 6 velocity) | fibers
+        self.fib0 = 3 * self.nper + 3 * int(self.boffs[-1]) + 6 * self.nbody
+        self.size = self.fib0 + 4 * self.npts
+        self.trg_fib = np.concatenate(self.fibers) if self.fibers else np.zeros((0, 3))
         self._setup(end_force)
 
     # -------------------------------------------------------------------
@@ -68,13 +86,37 @@ class HydroSystem:
         self.force = force
         # explicit flows (tstep.py:794-846): fibers -> fibers (self block
         # removed) and fibers -> shell
-        fw = force * np.concatenate(self.weights)[:, None]
+        fw = force * np.concatenate(self.weights)[:, None] if self.npts else np.zeros((0, 3))
         vfib2fib = self._fib2fib(fw)
         rhs = np.zeros(self.size)
+        # bodies: surface geometry, subtraction vectors, K
+        self.bnodes = np.concatenate([b[1] for b in self.bodies]) if self.nbody else np.zeros((0, 3))
+        self.bnrm = np.concatenate([b[2] for b in self.bodies]) if self.nbody else np.zeros((0, 3))
+        self.bw = np.concatenate([b[3] for b in self.bodies]) if self.nbody else np.zeros(0)
+        self.be = []
+        for (c, xb, nb, wb) in self.bodies:
+            self.be.append(P.singularity_subtraction_vectors(xb, nb, wb, eta))
+        # explicit flows of the external body force/torque: Stokeslet + rotlet at
+        # each centre (tstep.py:818-846), on fibers, bodies and shell
+        centres = np.array([b[0] for b in self.bodies]).reshape(-1, 3)
+        def body_explicit(trg):
+            if self.nbody == 0 or trg.shape[0] == 0:
+                return np.zeros(3 * trg.shape[0])
+            return (K.oseen_kernel_source_target(centres, trg, self.body_force[:, :3], eta)
+                    + K.rotlet_kernel_source_target(centres, trg, self.body_force[:, 3:], eta))
+        vfib2fib = vfib2fib + body_explicit(self.trg_fib).reshape(-1, 3)
+        if self.nbody:
+            vb = K.oseen_kernel_source_target(self.trg_fib, self.bnodes, fw, eta) \
+                if self.npts else np.zeros(3 * self.bnodes.shape[0])
+            vb = vb + body_explicit(self.bnodes)
+            for k in range(self.nbody):
+                o = self._body_off(k)
+                rhs[o:o + 3 * self.bsizes[k]] = -vb[3 * self.boffs[k]:3 * self.boffs[k + 1]]
         if self.shell is not None:
             xs_, ns_, ws_ = self.shell
-            vfib2shell = K.oseen_kernel_source_target(self.trg_fib, xs_, fw, eta)
-            rhs[:3 * self.nper] = -vfib2shell
+            vfib2shell = K.oseen_kernel_source_target(self.trg_fib, xs_, fw, eta) \
+                if self.npts else np.zeros(3 * self.nper)
+            rhs[:3 * self.nper] = -(vfib2shell + body_explicit(xs_))
             self.ex, self.ey, self.ez = P.singularity_subtraction_vectors(xs_, ns_, ws_, eta)
         # per-fiber square systems (get_fibers_and_bodies_matrices)
         self.A_sq = []
@@ -90,7 +132,7 @@ class HydroSystem:
                                flow=v_k)
             A_sq, b = OF.apply_BC_rectangular(fm, x, L, self.E, A, R, end_force=f_k[-1])
             self.A_sq.append(A_sq)
-            o = 3 * self.nper + 4 * self.offs[k]
+            o = self.fib0 + 4 * self.offs[k]
             rhs[o:o + 4 * N] = b
             # v_on_fib -> P_cheb [v; xs . D1 v], padded with 14 zero rows
             # (tstep.py:1150-1171)
@@ -109,10 +151,46 @@ class HydroSystem:
             xs_, ns_, ws_ = self.shell
             self.M_shell = P.shell_matrix(xs_, ns_, ws_, self.ex, self.ey, self.ez, eta)
             self.lu_shell = sla.lu_factor(self.M_shell)
+        # body preconditioner blocks [M - I_sub, -K; -K^T, I] (tstep_utils.py:1341-1387)
+        self.K = []
+        self.lu_body = []
+        for k, (c, xb, nb, wb) in enumerate(self.bodies):
+            Kb = self.k_matrix(xb, c)
+            self.K.append(Kb)
+            M = K.stresslet_kernel_times_normal(xb, nb, eta)
+            ex, ey, ez = self.be[k]
+            n = xb.shape[0]
+            for i in range(n):
+                M[3 * i:3 * i + 3, 3 * i + 0] -= ex[i] / wb[i]
+                M[3 * i:3 * i + 3, 3 * i + 1] -= ey[i] / wb[i]
+                M[3 * i:3 * i + 3, 3 * i + 2] -= ez[i] / wb[i]
+            A = np.zeros((3 * n + 6, 3 * n + 6))
+            A[:3 * n, :3 * n] = M
+            A[:3 * n, 3 * n:] = -Kb
+            A[3 * n:, :3 * n] = -Kb.T
+            A[3 * n:, 3 * n:] = np.eye(6)
+            self.lu_body.append(sla.lu_factor(A))
+
+    @staticmethod
+    def k_matrix(xb, centre):
+        """K = [J, rot] (3Nb x 6): K [U; w] = U + w x (r - centre) (body.py:162-193)."""
+        r = xb - centre
+        n = r.shape[0]
+        Kb = np.zeros((3 * n, 6))
+        for i in range(n):
+            Kb[3 * i:3 * i + 3, :3] = np.eye(3)
+            x, y, z = r[i]
+            Kb[3 * i:3 * i + 3, 3:] = [[0.0, z, -y], [-z, 0.0, x], [y, -x, 0.0]]
+        return Kb
+
+    def _body_off(self, k):
+        return 3 * self.nper + 3 * int(self.boffs[k]) + 6 * k
 
     def _fib2fib(self, fw):
         """Oseen flow of fiber forces on fiber nodes minus each fiber's self block
         (flow_fibers + self_flow_fibers, tstep_utils.py:653-729, :812-851)."""
+        if self.npts == 0:
+            return np.zeros((0, 3))
         v = K.oseen_kernel_source_target(self.trg_fib, self.trg_fib, fw, self.eta).reshape(-1, 3)
         for k, x in enumerate(self.fibers):
             o0, o1 = self.offs[k], self.offs[k + 1]
@@ -123,12 +201,23 @@ class HydroSystem:
     # -------------------------------------------------------------------
     def split(self, x):
         q = x[:3 * self.nper].reshape(-1, 3)
-        XT = x[3 * self.nper:]
+        XT = x[self.fib0:]
         return q, XT
 
+    def body_parts(self, x):
+        lam, vel = [], []
+        for k in range(self.nbody):
+            o = self._body_off(k)
+            n = self.bsizes[k]
+            lam.append(x[o:o + 3 * n].reshape(-1, 3))
+            vel.append(x[o + 3 * n:o + 3 * n + 6])
+        return lam, vel
+
     def matvec(self, x):
-        """A_body_fiber_hyd for fibers + shell (tstep.py:1003-1178)."""
+        """A_body_fiber_hyd (tstep.py:1003-1178) for fibers, free bodies and shell."""
         q, XT = self.split(x)
+        lam, vel = self.body_parts(x)
+        eta = self.eta
         res = np.zeros_like(x)
         f = np.zeros((self.npts, 3))
         for k in range(len(self.fibers)):
@@ -136,35 +225,66 @@ class HydroSystem:
             o = 4 * self.offs[k]
             fk = self.F_blocks[k] @ XT[o:o + 4 * N]
             f[self.offs[k]:self.offs[k + 1]] = fk.reshape(3, N).T
-        fw = f * np.concatenate(self.weights)[:, None]
+        fw = f * np.concatenate(self.weights)[:, None] if self.npts else f
         v_fib = self._fib2fib(fw)
+        lam_all = np.concatenate(lam) if self.nbody else np.zeros((0, 3))
+        if self.nbody:
+            # body double layer on every target (the j = i term is exactly zero)
+            if self.npts:
+                v_fib = v_fib + K.stresslet_kernel_source_target(
+                    self.bnodes, self.trg_fib, self.bnrm, lam_all, eta).reshape(-1, 3)
+            vb = K.stresslet_kernel_source_target(self.bnodes, self.bnodes, self.bnrm, lam_all, eta)
+            if self.npts:
+                vb = vb + K.oseen_kernel_source_target(self.trg_fib, self.bnodes, fw, eta)
         if self.shell is not None:
             xs_, ns_, ws_ = self.shell
-            vfib2shell = K.oseen_kernel_source_target(self.trg_fib, xs_, fw, self.eta)
-            res[:3 * self.nper] = self.M_shell @ q.ravel() + vfib2shell
-            v_fib = v_fib + K.stresslet_kernel_source_target(xs_, self.trg_fib, ns_, q,
-                                                             self.eta).reshape(-1, 3)
+            sh = self.M_shell @ q.ravel()
+            if self.npts:
+                sh = sh + K.oseen_kernel_source_target(self.trg_fib, xs_, fw, eta)
+                v_fib = v_fib + K.stresslet_kernel_source_target(xs_, self.trg_fib, ns_, q,
+                                                                 eta).reshape(-1, 3)
+            if self.nbody:
+                sh = sh + K.stresslet_kernel_source_target(self.bnodes, xs_, self.bnrm, lam_all, eta)
+                vb = vb + K.stresslet_kernel_source_target(xs_, self.bnodes, ns_, q, eta)
+            res[:3 * self.nper] = sh
+        for k in range(self.nbody):
+            o = self._body_off(k)
+            n = self.bsizes[k]
+            wb = self.bodies[k][3]
+            ex, ey, ez = self.be[k]
+            l = lam[k]
+            sub = (l[:, 0] / wb)[:, None] * ex + (l[:, 1] / wb)[:, None] * ey \
+                + (l[:, 2] / wb)[:, None] * ez
+            res[o:o + 3 * n] = vb[3 * self.boffs[k]:3 * self.boffs[k + 1]] - sub.ravel() \
+                - self.K[k] @ vel[k]
+            res[o + 3 * n:o + 3 * n + 6] = -self.K[k].T @ l.ravel() + vel[k]
         for k in range(len(self.fibers)):
             N = self.sizes[k]
             o = 4 * self.offs[k]
             vk = v_fib[self.offs[k]:self.offs[k + 1]].T.ravel()
-            res[3 * self.nper + o:3 * self.nper + o + 4 * N] = \
+            res[self.fib0 + o:self.fib0 + o + 4 * N] = \
                 self.A_sq[k] @ XT[o:o + 4 * N] - self.Q[k] @ vk
         return res
 
     def precond(self, x):
-        """P_inv (tstep.py:950-990): shell LU solve + per-fiber LU solves."""
+        """P_inv (tstep.py:950-990): shell LU, body LU, per-fiber LU solves."""
         y = np.empty_like(x)
         if self.shell is not None:
             y[:3 * self.nper] = sla.lu_solve(self.lu_shell, x[:3 * self.nper])
+        for k in range(self.nbody):
+            o = self._body_off(k)
+            m = 3 * self.bsizes[k] + 6
+            y[o:o + m] = sla.lu_solve(self.lu_body[k], x[o:o + m])
         for k in range(len(self.fibers)):
             N = self.sizes[k]
-            o = 3 * self.nper + 4 * self.offs[k]
+            o = self.fib0 + 4 * self.offs[k]
             y[o:o + 4 * N] = sla.lu_solve(self.lu_fib[k], x[o:o + 4 * N])
         return y
 
     def dense_operator(self):
-        """The full system matrix assembled from its blocks (independent of matvec)."""
+        """The full system matrix assembled from its blocks (independent of matvec).
+        Fibers and shell only."""
+        assert self.nbody == 0, "dense_operator covers fibers and shell"
         n = self.size
         A = np.zeros((n, n))
         W = np.concatenate(self.weights)

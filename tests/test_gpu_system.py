@@ -103,3 +103,31 @@ def test_block_jacobi_shell_preconditioner_same_solution():
     print("iterations dense / block:", r1.iterations, r2.iterations)
     assert r1.info == 0 and r2.info == 0
     assert rel2(r2.x.cpu().numpy(), r1.x.cpu().numpy()) < 1e-7
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-10), ("f32c", 1e-6)])
+def test_bodies_matvec_precond_and_solve(precision, tol):
+    """Fibers + 2 free rigid spheres + shell against the oracle."""
+    print(f"case bodies {precision}", flush=True)
+    fibers = configs.make_fibers(configs.FiberSpec(6, 16, 1.0, 2.0), seed=9)
+    bodies = configs.make_bodies(2, 0.5, 120, 2.2, fibers=fibers, seed=1)
+    force = np.array([[0.3, -0.2, 0.5, 0.1, 0.0, -0.2], [0.0, 0.4, -0.1, 0.0, 0.3, 0.1]])
+    xs, ns, ws = configs.make_periphery(300, 4.0)
+    ora = OT.HydroSystem(fibers, (xs, ns, ws), bodies=bodies, body_force=force)
+    gpu = FiberSystem(fibers, Shell(xs, ns, ws), bodies=bodies, body_force=force,
+                      pair_precision=precision)
+    torch.cuda.synchronize()
+    assert rel2(gpu.rhs.cpu().numpy(), ora.rhs) < 1e-12
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=ora.size)
+    y = gpu.matvec(torch.from_numpy(x).cuda())
+    z = gpu.precond(torch.from_numpy(x).cuda())
+    torch.cuda.synchronize()
+    assert rel2(y.cpu().numpy(), ora.matvec(x)) < tol
+    assert rel2(z.cpu().numpy(), ora.precond(x)) < 1e-10
+    if precision == "f64":
+        res = gmres(gpu.matvec, gpu.rhs, gpu.precond, tol=1e-10)
+        xo, info, _ = OT.gmres(ora.matvec, ora.rhs, ora.precond, tol=1e-10)
+        torch.cuda.synchronize()
+        assert res.info == 0 and info == 0
+        assert rel2(res.x.cpu().numpy(), xo) < 1e-7

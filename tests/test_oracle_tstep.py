@@ -54,3 +54,29 @@ def test_single_fiber_no_shell_reduces_to_block():
     S = OT.HydroSystem(configs.make_fibers(spec, seed=3))
     x = np.random.default_rng(2).normal(size=S.size)
     assert rel2(S.matvec(x), S.A_sq[0] @ x) < 1e-14
+
+
+def test_lone_sphere_obeys_stokes_law():
+    """A free sphere under an external force F and torque L translates at
+    F/(6 pi eta a) and rotates at L/(8 pi eta a^3): pins the body rows
+    (double layer, singularity subtraction, K) and the Stokeslet + rotlet
+    completion at the centre."""
+    a, eta = 0.5, 1.3
+    body = configs.make_sphere_body([0.3, -0.2, 0.1], a, 800)
+    F = np.array([0.7, -0.4, 1.1])
+    Lt = np.array([-0.5, 0.9, 0.2])
+    S = OT.HydroSystem([], None, eta=eta, bodies=[body], body_force=[np.concatenate([F, Lt])])
+    x, info, it = OT.gmres(S.matvec, S.rhs, S.precond, tol=1e-10)
+    assert info == 0
+    _, vel = S.body_parts(x)
+    U, W = vel[0][:3], vel[0][3:]
+    assert rel2(U, F / (6 * np.pi * eta * a)) < 1e-2
+    assert rel2(W, Lt / (8 * np.pi * eta * a**3)) < 1e-2
+
+
+def test_body_preconditioner_inverts_its_block():
+    body = configs.make_sphere_body([0.0, 0.0, 0.0], 0.5, 200)
+    S = OT.HydroSystem([], None, bodies=[body])
+    x = np.random.default_rng(4).normal(size=S.size)
+    # the body block alone is the whole operator here
+    assert rel2(S.precond(S.matvec(x)), x) < 1e-9
