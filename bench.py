@@ -53,7 +53,7 @@ def cpu_baseline(x, f, n_targets=4096, cores=None):
             "sample": f"Oseen FP64, all {N_PTS} sources x {n_targets} targets, median of 3"}
 
 
-def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto"):
+def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_bodies=True):
     """time-steps/s of the implicit fiber step, plain FP64 vs fast (+ learned guess).
 
     Each mode runs its own trajectory from the same seeded initial state:
@@ -76,9 +76,14 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto"):
     fibers = configs.make_fibers(spec, cfg.seed)
     xs, ns, ws = configs.make_periphery(nper, cfg.periphery_radius)
     shell = Shell(xs, ns, ws, eta=cfg.eta, pc=shell_pc, pc_f32=True)
+    bodies = None
+    if cfg_id == 4 and with_bodies:
+        # 8 free rigid spheres, radius 0.5, 600 nodes, no overlap (margin 0.1)
+        bodies = configs.make_bodies(8, 0.5, 600, spec.ball_radius, fibers=fibers, seed=cfg.seed)
     out = {"workload": f"{cfg.name}: {len(fibers)} fibers x {spec.num_points} nodes, "
-                       f"{nper}-node periphery, dt={cfg.dt}, GMRES tol {cfg.gmres_tol}"
-                       + (" (rigid bodies not included)" if cfg_id == 4 else ""),
+                       f"{nper}-node periphery"
+                       + (f", {len(bodies)} rigid spheres x 600 nodes" if bodies else "")
+                       + f", dt={cfg.dt}, GMRES tol {cfg.gmres_tol}",
            "shell_preconditioner": shell.pc}
     modes = [("plain", "plain", None), ("fast", "fast", None)]
     if train_steps:
@@ -87,7 +92,8 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto"):
     for name, mode, pred in modes:
         ts = TimeStepper(fibers, shell, mode=mode, eta=cfg.eta, dt=cfg.dt, E=spec.E,
                          epsilon=spec.epsilon, tol=cfg.gmres_tol, restart=cfg.restart,
-                         maxiter=cfg.maxiter, predictor=pred, lengths=np.full(len(fibers), spec.length))
+                         maxiter=cfg.maxiter, predictor=pred, lengths=np.full(len(fibers), spec.length),
+                         bodies=bodies)
         # the learned mode collects samples from its 4th step on and trains
         # after train_steps of them: all of that happens before timing starts
         warm = train_steps + 4 if pred is not None else 1
@@ -212,18 +218,18 @@ def main():
         for _ in range(warmup):
             FK.pairsum(kernel, X, den, X, src_nrm=nrm_t, excl=excl, precision=precision, out=out)
         torch.cuda.synchronize()
-        stream = torch.cuda.current_stream()
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(steps)]
+        # HIP events around the pair kernel alone, recorded by libf3d on the
+        # stream it launches on (the caller's current stream)
+        FK.kernel_timer(True)
         t0 = time.perf_counter()
-        for s, e in ev:
-            s.record(stream)
+        for _ in range(steps):
             FK.pairsum(kernel, X, den, X, src_nrm=nrm_t, excl=excl, precision=precision, out=out)
-            e.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        kms = float(np.mean([s.elapsed_time(e) for s, e in ev]))
-        return wall / steps * 1e3, kms, out.clone()
+        kms, launches = FK.kernel_timer_read()
+        FK.kernel_timer(False)
+        assert launches == steps, (launches, steps)
+        return wall / steps * 1e3, kms / launches, out.clone()
 
     # headline: FP32-compensated Oseen
     ms32, k32, u32 = run("oseen", "f32c", args.steps, args.warmup)

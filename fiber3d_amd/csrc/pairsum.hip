@@ -493,6 +493,16 @@ static int env_int(const char* name, int dflt) {
 static int g_launch_k[3][2] = {{2, 2}, {2, 2}, {2, 2}};
 static int g_launch_waves[3][2] = {{8192, 8192}, {8192, 8192}, {8192, 8192}};
 
+// Event pairs around the pair kernel, recorded while f3d_pairsum_timer is on.
+constexpr int kTimerSlots = 4096;
+struct KernelTimer {
+  bool on = false;
+  bool made = false;
+  int n = 0;
+  hipEvent_t ev[2 * kTimerSlots];
+};
+static KernelTimer g_timer;
+
 static Plan make_plan(int kernel, int precision, int64_t ns, int64_t nt) {
   Plan pl;
   const int kk = (kernel >= 0 && kernel < 3) ? kernel : 0;
@@ -565,6 +575,8 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
 
   const dim3 grid((unsigned)ceil_div(nt, (int64_t)kBlock * pl.K), (unsigned)pl.P);
   const bool clamp = kQuant && (flags & F3D_CLAMP_NEAR);
+  const bool timed = g_timer.on && g_timer.n < kTimerSlots;
+  if (timed) F3D_HIP_CHECK(hipEventRecord(g_timer.ev[2 * g_timer.n], st));
 #define F3D_PS(KK, CL)                                                                        \
   F3D_LAUNCH((pairsum_kernel<T, KIND, KK, CL>), grid, dim3(kBlock), 0, st, rec, pl.chunk,     \
              pl.ns_pad, tpos, (long)t_sp, (long)t_sc, (int)nt, lo, hi, reg, eps,              \
@@ -579,6 +591,10 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
     else F3D_PS(4, false);
   }
 #undef F3D_PS
+  if (timed) {
+    F3D_HIP_CHECK(hipEventRecord(g_timer.ev[2 * g_timer.n + 1], st));
+    ++g_timer.n;
+  }
 
   double factor;
   if (KIND == F3D_STRESSLET) factor = -3.0 / (4.0 * kPi * eta);
@@ -660,6 +676,33 @@ extern "C" int f3d_pairsum_set_launch(int kernel, int precision, int32_t k,
   const int pp = precision == F3D_F64 ? 0 : 1;
   g_launch_k[kernel][pp] = k ? k : 2;
   g_launch_waves[kernel][pp] = target_waves ? target_waves : 8192;
+  return F3D_OK;
+}
+
+extern "C" int f3d_pairsum_timer(int enable) {
+  if (enable && !g_timer.made) {
+    for (int i = 0; i < 2 * kTimerSlots; ++i) F3D_HIP_CHECK(hipEventCreate(&g_timer.ev[i]));
+    g_timer.made = true;
+  }
+  g_timer.on = enable != 0;
+  if (enable) g_timer.n = 0;
+  return F3D_OK;
+}
+
+extern "C" int f3d_pairsum_timer_read(double* total_ms, int32_t* launches) {
+  F3D_REQUIRE(total_ms != nullptr && launches != nullptr, F3D_EINVAL,
+              "timer_read: NULL output");
+  double sum = 0.0;
+  if (g_timer.made && g_timer.n > 0) {
+    F3D_HIP_CHECK(hipEventSynchronize(g_timer.ev[2 * g_timer.n - 1]));
+    for (int i = 0; i < g_timer.n; ++i) {
+      float ms = 0.f;
+      F3D_HIP_CHECK(hipEventElapsedTime(&ms, g_timer.ev[2 * i], g_timer.ev[2 * i + 1]));
+      sum += ms;
+    }
+  }
+  *total_ms = sum;
+  *launches = g_timer.made ? g_timer.n : 0;
   return F3D_OK;
 }
 

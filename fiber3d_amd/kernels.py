@@ -11,6 +11,8 @@ float64 CUDA tensors in and out, no host round trip.
 A module-level precision (``set_precision``) selects FP64 (default, parity
 path) or FP32-compensated evaluation for the drop-in names.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -62,7 +64,6 @@ def _stream():
 
 
 def ctypes_stream(s):
-    import ctypes
     return ctypes.c_void_p(s.cuda_stream)
 
 
@@ -148,6 +149,22 @@ def periphery_apply(pos, nrm, w, ex, ey, ez, q, add=None, eta=1.0, reg=5e-3, eps
                                  float(eps), L.ptr(out), L.ptr(ws), ws.numel(), _stream())
     L.check(rc, "f3d_periphery_apply")
     return out
+
+
+def kernel_timer(enable=True):
+    """Start (clearing earlier records) or stop HIP-event timing of the pair
+    kernel inside every later pairsum / periphery_apply call."""
+    L.check(L.load().f3d_pairsum_timer(1 if enable else 0), "f3d_pairsum_timer")
+
+
+def kernel_timer_read():
+    """(summed milliseconds, launch count) of the pair-kernel launches timed
+    since kernel_timer(True); waits for the last one."""
+    ms = ctypes.c_double()
+    n = ctypes.c_int32()
+    L.check(L.load().f3d_pairsum_timer_read(ctypes.byref(ms), ctypes.byref(n)),
+            "f3d_pairsum_timer_read")
+    return ms.value, n.value
 
 
 # ---------------------------------------------------------------------------

@@ -64,6 +64,19 @@ int f3d_abi_version(void);
  */
 int f3d_pairsum_set_launch(int kernel, int precision, int32_t k, int32_t target_waves);
 
+/*
+ * Timing of the pair kernel alone (not the bounding-box, pack and reduce
+ * launches around it): enable != 0 clears the record and makes every later
+ * f3d_pairsum / f3d_periphery_apply record a HIP event pair around its pair
+ * kernel on the caller's stream (at most 4096 launches are kept); enable == 0
+ * stops recording.  f3d_pairsum_timer_read waits for the last recorded launch
+ * and returns the summed milliseconds and the launch count.  Process-wide,
+ * one device, not thread safe.  Instrumentation for benchmarks and profilers;
+ * no reference counterpart.
+ */
+int f3d_pairsum_timer(int enable);
+int f3d_pairsum_timer_read(double* total_ms, int32_t* launches);
+
 /* Bytes of workspace f3d_pairsum needs for these sizes. */
 size_t f3d_pairsum_workspace_size(int kernel, int precision, int64_t ns, int64_t nt);
 
