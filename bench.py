@@ -175,6 +175,17 @@ def lu_leg(reps=5, cpu=True):
     return out
 
 
+def copy_probe(mib):
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_launch": {"read": a.numel() * 4,
+                                                      "write": a.numel() * 4}, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,7 +198,13 @@ def main():
     ap.add_argument("--train-steps", type=int, default=0)
     ap.add_argument("--lu", action="store_true", help="also time the batched fiber LU (workload 2)")
     ap.add_argument("--shell-pc", default="auto", choices=["auto", "dense", "block"])
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 plain device copies of MIB MiB (calibrates the HBM "
+                         "byte counters FETCH_SIZE / WRITE_SIZE against a known size)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
               "the multi-GPU sharding is a design in DESIGN.md, not a measured path.",

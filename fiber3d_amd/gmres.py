@@ -15,7 +15,11 @@ rotations on the host (j+2 scalars per iteration).
 
 An optional `inner` operator (e.g. the FP32-compensated matvec) is used for
 the Arnoldi products while the true residual that decides convergence and
-restarts is always recomputed with the FP64 `matvec`.
+restarts is always recomputed with the FP64 `matvec`.  The inner operator's
+error is relative to the size of the vector it acts on, so it limits how far
+one cycle can reduce the true residual: when a cycle converges by its own
+estimate but the FP64 residual is still above tol, the inner operator has
+reached its accuracy floor and the remaining cycles use `matvec`.
 """
 import math
 from dataclasses import dataclass
@@ -33,18 +37,20 @@ class GmresResult:
     residual: float          # true ||b - A x|| / ||b|| in FP64
     guess_used: bool
     history: list
+    inner_cycles: int = 0    # restart cycles run with the inner operator
 
 
 def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, inner=None,
           callback=None):
     Pinv = precond if precond is not None else (lambda v, out=None: v.clone())
     op = inner if inner is not None else matvec
+    inner_cycles = 0
     n = b.numel()
     dev = b.device
     bnorm = float(torch.linalg.norm(b))
     hist = []
     if bnorm == 0.0:
-        return GmresResult(torch.zeros_like(b), 0, 0, 0, 0.0, False, hist)
+        return GmresResult(torch.zeros_like(b), 0, 0, 0, 0.0, False, hist, 0)
     guess_used = False
     x = torch.zeros_like(b)
     r = b.clone()
@@ -65,7 +71,8 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
         beta = float(torch.linalg.norm(r))
         hist.append(beta / bnorm)
         if beta <= tol * bnorm:
-            return GmresResult(x, 0, total, restarts, beta / bnorm, guess_used, hist)
+            return GmresResult(x, 0, total, restarts, beta / bnorm, guess_used, hist,
+                               inner_cycles)
         V[0] = r / beta
         H = np.zeros((m + 1, m))
         cs = np.zeros(m)
@@ -110,5 +117,10 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
         x += Pinv(upd)
         r = b - matvec(x)
         restarts += 1
+        if op is not matvec:
+            inner_cycles += 1
+            if est <= tol and float(torch.linalg.norm(r)) > tol * bnorm:
+                op = matvec
     rn = float(torch.linalg.norm(r)) / bnorm
-    return GmresResult(x, 0 if rn <= tol else total, total, restarts, rn, guess_used, hist)
+    return GmresResult(x, 0 if rn <= tol else total, total, restarts, rn, guess_used, hist,
+                       inner_cycles)

@@ -53,13 +53,16 @@ class Shell:
     the shell block of the preconditioner.
 
     pc='dense' builds the reference's dense operator (tstep.py:328-345) and
-    LU-factors it once (FP64 as in the reference; FP32 as well when
-    pc_f32=True, for the fast mode).  pc='block' inverts only each node's 3x3
-    diagonal block, -E_i/w_i - I/w_i + n_i n_i^T: the dense factor of a
-    32768-node shell would be 98304^2 (77 GB in FP64), so large shells use
-    this instead.  pc='auto' picks 'dense' up to 12000 nodes."""
+    LU-factors it once (tstep.py:346-351; FP64, plus an FP32 factor when
+    pc_f32=True for the fast mode).  pc='block' inverts only each node's 3x3
+    diagonal block, -E_i/w_i - I/w_i + n_i n_i^T.  On the GPU the two
+    triangular solves of a 3N x 3N factor are latency-bound sweeps (~60 ms per
+    apply at 8000 nodes, against ~2 ms for a whole matvec), while the block
+    inverse costs 0.1 ms per apply and about doubles the GMRES iteration count,
+    and the dense factor of a 32768-node shell would be 98304^2 (77 GB in
+    FP64); pc='auto' therefore picks 'dense' only up to 2048 nodes."""
 
-    DENSE_MAX = 12000
+    DENSE_MAX = 2048
 
     def __init__(self, x, normals, weights, eta=1.0, reg=5e-3, eps=1e-5, pc="auto",
                  pc_f32=False):
@@ -134,8 +137,16 @@ class Bodies:
     """Free rigid bodies on the GPU: surface nodes, outward normals, weights,
     singularity-subtraction vectors, K = [J, rot] (body/body.py:162-193,
     :325-346) and the dense per-body preconditioner block
-    [M - I_sub, -K; -K^T, I] (tstep_utils.py:1341-1387), LU-factored once per
-    step.  All bodies must carry the same node count (batched factor)."""
+    A = [M - I_sub, -K; -K^T, I] (tstep_utils.py:1341-1387).
+
+    The preconditioner follows the reference's precompute option
+    (tstep.py:225-252, :966-980): A is inverted once, in each body's initial
+    pose, and applied in the body frame.  A rigid rotation R of a body maps
+    its block to Q A0 Q^T with Q = diag(R per node, R, R) (the stresslet is a
+    rank-3 tensor, K picks up R on both sides), so A^-1 = Q A0^-1 Q^T: every
+    3-vector of the body's slice is rotated by R^T, multiplied by A0^-1
+    (one batched FP64 GEMV) and rotated back.  All bodies must carry the same
+    node count (batched apply)."""
 
     def __init__(self, bodies, eta, reg, eps, body_force=None):
         dev = device()
@@ -155,10 +166,29 @@ class Bodies:
         f = np.zeros((self.count, 6)) if body_force is None else np.asarray(body_force, float)
         self.force = t(f)
         self.vb = torch.zeros((self.n, 3), dtype=torch.float64, device=dev)
+        # rotation of every body from the pose its inverse block was built in
+        self.rot = torch.eye(3, dtype=torch.float64, device=dev).repeat(self.count, 1, 1)
         self.update_geometry()
+        self.ainv = self._invert(self.pc_blocks())
+
+    @staticmethod
+    def _invert(A, panel=64):
+        """Inverse of each (m, m) block of A from its LU factors, solved against
+        identity panels of `panel` columns (bounds the triangular-solve
+        workspace hipBLAS allocates)."""
+        out = torch.empty_like(A)
+        m = A.shape[-1]
+        for k in range(A.shape[0]):
+            lu, piv = torch.linalg.lu_factor(A[k])
+            for c0 in range(0, m, panel):
+                w = min(panel, m - c0)
+                e = torch.zeros((m, w), dtype=A.dtype, device=A.device)
+                e[c0 + torch.arange(w, device=A.device), torch.arange(w, device=A.device)] = 1.0
+                out[k, :, c0:c0 + w] = torch.linalg.lu_solve(lu, piv, e)
+        return out
 
     def update_geometry(self):
-        """Subtraction vectors and the preconditioner for the current pose."""
+        """Centre-relative nodes and subtraction vectors for the current pose."""
         nb = self.nb
         self.r = self.pos - self.centres[self.bid]
         self.e = []
@@ -171,6 +201,10 @@ class Bodies:
                 pairsum("stresslet", self.pos[s], dens[s], self.pos[s], src_nrm=self.nrm[s],
                         eta=self.eta, reg=self.reg, eps=self.eps, out=e[s])
             self.e.append(e)
+
+    def pc_blocks(self):
+        """The dense blocks [M - I_sub, -K; -K^T, I] of the current pose."""
+        nb = self.nb
         m = 3 * nb + 6
         A = torch.zeros((self.count, m, m), dtype=torch.float64, device=self.pos.device)
         for k in range(self.count):
@@ -189,7 +223,7 @@ class Bodies:
             A[k, :3 * nb, 3 * nb:] = -Kb
             A[k, 3 * nb:, :3 * nb] = -Kb.t()
             A[k, 3 * nb:, 3 * nb:] = torch.eye(6, dtype=torch.float64, device=A.device)
-        self.lu, self.piv = torch.linalg.lu_factor(A)
+        return A
 
     @staticmethod
     def k_matrix(r):
@@ -247,8 +281,11 @@ class Bodies:
         ob[:, 3 * self.nb + 3:] = vel[:, 3:] - T
 
     def precond(self, x, out, start):
-        xb = self.block(x, start)
-        self.block(out, start).copy_(torch.linalg.lu_solve(self.lu, self.piv, xb[..., None])[..., 0])
+        xb = self.block(x, start).view(self.count, self.nb + 2, 3)
+        # rows are 3-vectors v^T: (R^T v)^T = v^T R, and back (R y)^T = y^T R^T
+        yb = torch.bmm(self.ainv, torch.bmm(xb, self.rot).reshape(self.count, -1, 1))
+        yb = torch.bmm(yb.view(self.count, self.nb + 2, 3), self.rot.transpose(1, 2).contiguous())
+        self.block(out, start).copy_(yb.reshape(self.count, -1))
 
     def move(self, vel, dt):
         """Rigid update: centre += U dt, nodes rotated by Omega dt (Rodrigues)."""
@@ -256,6 +293,13 @@ class Bodies:
         th = torch.linalg.norm(W, dim=1) * dt
         ax = W / torch.linalg.norm(W, dim=1, keepdim=True).clamp_min(1e-300)
         c, s = torch.cos(th)[:, None], torch.sin(th)[:, None]
+        # R = c I + s [k]_x + (1 - c) k k^T per body, composed onto self.rot
+        kx = torch.zeros((self.count, 3, 3), dtype=torch.float64, device=W.device)
+        kx[:, 0, 1], kx[:, 0, 2], kx[:, 1, 2] = -ax[:, 2], ax[:, 1], -ax[:, 0]
+        kx = kx - kx.transpose(1, 2)
+        Rinc = c[:, :, None] * torch.eye(3, dtype=torch.float64, device=W.device) \
+            + s[:, :, None] * kx + (1 - c)[:, :, None] * ax[:, :, None] * ax[:, None, :]
+        self.rot = torch.bmm(Rinc, self.rot)
         def rot(v):
             k = ax[self.bid]
             cc, ss = c[self.bid], s[self.bid]
@@ -314,6 +358,11 @@ class FiberSystem:
         lo = np.repeat(self.offs[:-1], sizes).astype(np.int32)
         hi = np.repeat(self.offs[1:], sizes).astype(np.int32)
         self.excl = (torch.from_numpy(lo).to(self.dev), torch.from_numpy(hi).to(self.dev))
+        # plus-end node of every fiber, and for fiber targets the exclusion of
+        # their own fiber's end node in the list of end nodes
+        self.end_idx = torch.from_numpy(self.offs[1:] - 1).to(self.dev)
+        fid = np.repeat(np.arange(self.nfib), sizes).astype(np.int32)
+        self.excl_end = (torch.from_numpy(fid).to(self.dev), torch.from_numpy(fid + 1).to(self.dev))
         # block layouts
         self.bs = BlockSet(4 * sizes)  # A_sq blocks (4N x 4N)
         fo = self.offs[:-1]
@@ -423,13 +472,16 @@ class FiberSystem:
         """RHS: explicit flows of the plus-end forces and per-fiber RHS
         (tstep.py:794-846, tstep_utils.py:1238-1336, fiber.py:473-582)."""
         dev = self.dev
-        # explicit force: plus end along the end tangent
+        # explicit force: plus end along the end tangent.  Only the end nodes
+        # carry force, so only they are sources of the explicit flows (the
+        # zero-force nodes add exact zeros to the reference's all-node sums).
         force = torch.zeros((self.npts, 3), dtype=torch.float64, device=dev)
-        last = torch.from_numpy(self.offs[1:] - 1).to(dev)
+        last = self.end_idx
         force[last] = self.end_force * self.xs[last]
         self.force = force
+        Xe, fe, we = self.X[last], force[last], self.weights[last]
         pp = self.pair_precision_rhs = "f64"
-        vff = pairsum("oseen", self.X, force, self.X, src_w=self.weights, excl=self.excl,
+        vff = pairsum("oseen", Xe, fe, self.X, src_w=we, excl=self.excl_end,
                       eta=self.eta, reg=self.reg, eps=self.eps, precision=pp)
         self.v_explicit = vff  # explicit flow on fiber nodes (feature of the learned guess)
         rhs = torch.zeros(self.size, dtype=torch.float64, device=dev)
@@ -437,12 +489,12 @@ class FiberSystem:
         if B is not None:
             # external body force/torque: Stokeslet + rotlet at each centre
             B.explicit(self.X, out=vff, accumulate=True)
-            vb = pairsum("oseen", self.X, force, B.pos, src_w=self.weights, eta=self.eta,
+            vb = pairsum("oseen", Xe, fe, B.pos, src_w=we, eta=self.eta,
                          reg=self.reg, eps=self.eps, precision=pp)
             B.explicit(B.pos, out=vb, accumulate=True)
             B.scatter_density(rhs, -vb, 3 * self.nper)
         if self.shell is not None:
-            vfs = pairsum("oseen", self.X, force, self.shell.pos, src_w=self.weights,
+            vfs = pairsum("oseen", Xe, fe, self.shell.pos, src_w=we,
                           eta=self.eta, reg=self.reg, eps=self.eps, precision=pp)
             if B is not None:
                 B.explicit(self.shell.pos, out=vfs, accumulate=True)

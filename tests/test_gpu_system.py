@@ -131,3 +131,27 @@ def test_bodies_matvec_precond_and_solve(precision, tol):
         torch.cuda.synchronize()
         assert res.info == 0 and info == 0
         assert rel2(res.x.cpu().numpy(), xo) < 1e-7
+
+
+def test_bodies_precond_after_rigid_motion():
+    """The body inverse built in the initial pose and rotated with the bodies
+    equals the preconditioner the oracle factors afresh at the moved pose."""
+    print("case bodies moved", flush=True)
+    fibers = configs.make_fibers(configs.FiberSpec(4, 16, 1.0, 2.0), seed=3)
+    bodies = configs.make_bodies(3, 0.5, 120, 2.2, fibers=fibers, seed=4)
+    xs, ns, ws = configs.make_periphery(300, 4.0)
+    gpu = FiberSystem(fibers, Shell(xs, ns, ws), bodies=bodies)
+    rng = np.random.default_rng(11)
+    for _ in range(2):
+        vel = torch.from_numpy(rng.normal(size=(3, 6))).cuda()
+        gpu.bodies.move(vel, 0.25)
+    B = gpu.bodies
+    nb = B.nb
+    moved = [(B.centres[k].cpu().numpy(), B.pos[k * nb:(k + 1) * nb].cpu().numpy(),
+              B.nrm[k * nb:(k + 1) * nb].cpu().numpy(), B.w[k * nb:(k + 1) * nb].cpu().numpy())
+             for k in range(B.count)]
+    ora = OT.HydroSystem(fibers, (xs, ns, ws), bodies=moved)
+    x = rng.normal(size=ora.size)
+    z = gpu.precond(torch.from_numpy(x).cuda())
+    torch.cuda.synchronize()
+    assert rel2(z.cpu().numpy(), ora.precond(x)) < 1e-10
