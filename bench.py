@@ -30,6 +30,12 @@ OSEEN_FLOPS = 28      # per pair: 3 sub, r^2 5, rsqrt 1, d.f 5, r^-2 1, scale 1,
 STRESSLET_FLOPS = 30
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak (packed)
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes per launch of the FP32-compensated Oseen pair kernel at N = M = 65536:
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --steps 2
+# --warmup 1 --no-cpu` (17454.4 and 49152 KiB per dispatch), scaled by what the
+# same counters read for `bench.py --copy-probe 512` (FETCH_SIZE x 2.0, WRITE_SIZE
+# x 1.0).  Mostly the per-chunk partial sums the reduce kernel reads back.
+OSEEN_F32C_HBM_BYTES = 86076086
 
 
 def cpu_baseline(x, f, n_targets=4096, cores=None):
@@ -53,7 +59,8 @@ def cpu_baseline(x, f, n_targets=4096, cores=None):
             "sample": f"Oseen FP64, all {N_PTS} sources x {n_targets} targets, median of 3"}
 
 
-def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_bodies=True):
+def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_bodies=True,
+               graph=True):
     """time-steps/s of the implicit fiber step, plain FP64 vs fast (+ learned guess).
 
     Each mode runs its own trajectory from the same seeded initial state:
@@ -84,7 +91,7 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
                        f"{nper}-node periphery"
                        + (f", {len(bodies)} rigid spheres x 600 nodes" if bodies else "")
                        + f", dt={cfg.dt}, GMRES tol {cfg.gmres_tol}",
-           "shell_preconditioner": shell.pc}
+           "shell_preconditioner": shell.pc, "arnoldi_operator": "hip graph" if graph else "eager"}
     modes = [("plain", "plain", None), ("fast", "fast", None)]
     if train_steps:
         modes.append(("extrapolated_fast", "fast", LearnedGuess(train_steps=10**9, use_net=False)))
@@ -93,7 +100,7 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
         ts = TimeStepper(fibers, shell, mode=mode, eta=cfg.eta, dt=cfg.dt, E=spec.E,
                          epsilon=spec.epsilon, tol=cfg.gmres_tol, restart=cfg.restart,
                          maxiter=cfg.maxiter, predictor=pred, lengths=np.full(len(fibers), spec.length),
-                         bodies=bodies)
+                         bodies=bodies, graph=graph)
         # the learned mode collects samples from its 4th step on and trains
         # after train_steps of them: all of that happens before timing starts
         warm = train_steps + 4 if pred is not None else 1
@@ -198,6 +205,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=0)
     ap.add_argument("--lu", action="store_true", help="also time the batched fiber LU (workload 2)")
     ap.add_argument("--shell-pc", default="auto", choices=["auto", "dense", "block"])
+    ap.add_argument("--no-graph", action="store_true",
+                    help="run the Arnoldi operator eagerly instead of as a recorded HIP graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (calibrates the HBM "
                          "byte counters FETCH_SIZE / WRITE_SIZE against a known size)")
@@ -276,7 +285,8 @@ def main():
                    "eta": 1.0},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": OSEEN_F32C_HBM_BYTES, "traffic_unit": "bytes/launch",
                      "kernel": "pairsum_kernel<float,OSEEN>", "flops_per_pair": OSEEN_FLOPS,
                      "kernel_ms": round(k32, 4)},
         "oseen_f32c_rel_err_vs_f64": err32,
@@ -290,7 +300,7 @@ def main():
         line["batched_lu"] = lu_leg(cpu=not args.no_cpu)
     if args.system >= 0:
         line["fiber_system"] = system_leg(args.system, args.system_steps, args.train_steps,
-                                          shell_pc=args.shell_pc)
+                                          shell_pc=args.shell_pc, graph=not args.no_graph)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(x, f)
     print(json.dumps(line), flush=True)

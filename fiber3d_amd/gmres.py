@@ -20,6 +20,10 @@ error is relative to the size of the vector it acts on, so it limits how far
 one cycle can reduce the true residual: when a cycle converges by its own
 estimate but the FP64 residual is still above tol, the inner operator has
 reached its accuracy floor and the remaining cycles use `matvec`.
+
+`fused_inner` / `fused_matvec`, when given, compute v -> A P^-1 v in one call
+(e.g. a recorded HIP graph of the preconditioner and the operator) and replace
+the separate precond + operator calls of the Arnoldi step.
 """
 import math
 from dataclasses import dataclass
@@ -41,7 +45,7 @@ class GmresResult:
 
 
 def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, inner=None,
-          callback=None):
+          callback=None, fused_inner=None, fused_matvec=None):
     Pinv = precond if precond is not None else (lambda v, out=None: v.clone())
     op = inner if inner is not None else matvec
     inner_cycles = 0
@@ -80,9 +84,13 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
         g = np.zeros(m + 1)
         g[0] = beta
         j_done = 0
+        fused = fused_matvec if op is matvec else fused_inner
         for j in range(m):
-            Pinv(V[j], out=z)
-            op(z, out=w)
+            if fused is not None:
+                fused(V[j], out=w)
+            else:
+                Pinv(V[j], out=z)
+                op(z, out=w)
             # CGS2 against V[0..j]
             Vj = V[:j + 1]
             h1 = torch.mv(Vj, w)

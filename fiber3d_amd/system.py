@@ -44,6 +44,22 @@ def _scatter_blocks(buf, offs_h, blocks):
             buf[offs_h[b]:offs_h[b] + n].copy_(blocks[b])
 
 
+def dense_inverse(A, panel=64):
+    """Inverse of a square matrix from one LU factorisation, solved against
+    identity panels of `panel` columns: the triangular-solve workspace hipBLAS
+    allocates grows with the right-hand-side width and fails for thousands of
+    columns."""
+    lu, piv = torch.linalg.lu_factor(A)
+    m = A.shape[-1]
+    out = torch.empty_like(A)
+    for c0 in range(0, m, panel):
+        w = min(panel, m - c0)
+        e = torch.zeros((m, w), dtype=A.dtype, device=A.device)
+        e[c0 + torch.arange(w, device=A.device), torch.arange(w, device=A.device)] = 1.0
+        out[:, c0:c0 + w] = torch.linalg.lu_solve(lu, piv, e)
+    return out
+
+
 def _c0c1(epsilon, eta):
     return -np.log(np.e * epsilon**2) / (8.0 * np.pi * eta), 2.0 / (8.0 * np.pi * eta)
 
@@ -53,14 +69,15 @@ class Shell:
     the shell block of the preconditioner.
 
     pc='dense' builds the reference's dense operator (tstep.py:328-345) and
-    LU-factors it once (tstep.py:346-351; FP64, plus an FP32 factor when
-    pc_f32=True for the fast mode).  pc='block' inverts only each node's 3x3
-    diagonal block, -E_i/w_i - I/w_i + n_i n_i^T.  On the GPU the two
-    triangular solves of a 3N x 3N factor are latency-bound sweeps (~60 ms per
-    apply at 8000 nodes, against ~2 ms for a whole matvec), while the block
-    inverse costs 0.1 ms per apply and about doubles the GMRES iteration count,
-    and the dense factor of a 32768-node shell would be 98304^2 (77 GB in
-    FP64); pc='auto' therefore picks 'dense' only up to 2048 nodes."""
+    inverts it once; the reference LU-factors it (tstep.py:346-351) and
+    applies lu_solve, the same linear map.  The inverse is applied as one
+    GEMV, which a HIP graph can capture (rocSOLVER's getrs allocates) and which
+    is bandwidth-bound instead of a latency-bound pair of triangular sweeps.
+    pc_f32=True also keeps an FP32 copy for the fast mode.  pc='block'
+    inverts only each node's 3x3 diagonal block, -E_i/w_i - I/w_i + n_i n_i^T
+    (0.1 ms per apply at 32768 nodes; about twice the GMRES iterations of the
+    dense form).  The dense inverse grows as (3N)^2: 4.6 GB at 8000 nodes,
+    77 GB at 32768, so pc='auto' picks 'dense' only up to 2048 nodes."""
 
     DENSE_MAX = 2048
 
@@ -82,13 +99,11 @@ class Shell:
         if pc == "auto":
             pc = "dense" if self.n <= self.DENSE_MAX else "block"
         self.pc = pc
-        self.lu = self.lu32 = None
+        self.minv = self.minv32 = None
         if pc == "dense":
-            M = self.dense_matrix()
+            self.minv = dense_inverse(self.dense_matrix())
             if pc_f32:
-                self.lu32, self.piv32 = torch.linalg.lu_factor(M.float())
-            self.lu, self.piv = torch.linalg.lu_factor(M)
-            del M
+                self.minv32 = self.minv.float()
         elif pc == "block":
             E = torch.stack(self.e, dim=2)  # E[i, a, c] = e_c[i, a]
             winv = (1.0 / self.w)[:, None, None]
@@ -125,11 +140,10 @@ class Shell:
     def precond(self, r, out, f32=False):
         if self.pc == "block":
             out.view(self.n, 3).copy_(torch.matmul(self.Dinv, r.view(self.n, 3, 1))[..., 0])
-        elif f32 and self.lu32 is not None:
-            out.copy_(torch.linalg.lu_solve(self.lu32, self.piv32, r.float().reshape(-1, 1))
-                      .reshape(-1))
+        elif f32 and self.minv32 is not None:
+            out.copy_(torch.mv(self.minv32, r.float()))
         else:
-            out.copy_(torch.linalg.lu_solve(self.lu, self.piv, r.reshape(-1, 1)).reshape(-1))
+            torch.mv(self.minv, r, out=out)
         return out
 
 
@@ -172,20 +186,8 @@ class Bodies:
         self.ainv = self._invert(self.pc_blocks())
 
     @staticmethod
-    def _invert(A, panel=64):
-        """Inverse of each (m, m) block of A from its LU factors, solved against
-        identity panels of `panel` columns (bounds the triangular-solve
-        workspace hipBLAS allocates)."""
-        out = torch.empty_like(A)
-        m = A.shape[-1]
-        for k in range(A.shape[0]):
-            lu, piv = torch.linalg.lu_factor(A[k])
-            for c0 in range(0, m, panel):
-                w = min(panel, m - c0)
-                e = torch.zeros((m, w), dtype=A.dtype, device=A.device)
-                e[c0 + torch.arange(w, device=A.device), torch.arange(w, device=A.device)] = 1.0
-                out[k, :, c0:c0 + w] = torch.linalg.lu_solve(lu, piv, e)
-        return out
+    def _invert(A):
+        return torch.stack([dense_inverse(A[k]) for k in range(A.shape[0])])
 
     def update_geometry(self):
         """Centre-relative nodes and subtraction vectors for the current pose."""

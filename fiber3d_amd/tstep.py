@@ -22,6 +22,7 @@ import numpy as np
 import torch
 
 from .gmres import gmres
+from .graphs import ArnoldiGraphs
 from .system import FiberSystem, Shell
 
 
@@ -38,7 +39,7 @@ class TimeStepper:
     def __init__(self, fibers, shell, mode="plain", eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference", tol=1e-8,
                  restart=150, maxiter=200, predictor=None, lengths=None, bodies=None,
-                 body_force=None):
+                 body_force=None, graph=False):
         if mode not in ("plain", "fast"):
             raise ValueError("mode is 'plain' or 'fast'")
         self.mode = mode
@@ -48,7 +49,10 @@ class TimeStepper:
                                penalty=penalty, nfd=nfd, end_force=end_force,
                                discretisation=discretisation, lengths=lengths,
                                bodies=bodies, body_force=body_force,
-                               pair_precision="f32c" if fast else "f64",
+                               # the system's own operator (true residual,
+                               # convergence) is always FP64; the fast mode
+                               # adds the FP32-compensated inner operator
+                               pair_precision="f64",
                                lu_precision="f32c" if fast else "f64",
                                # a preconditioner need not be exact: the FP32
                                # factors are applied without refinement (a fixed
@@ -56,6 +60,8 @@ class TimeStepper:
                                lu_refine=0)
         self.predictor = predictor
         S = self.sys
+        # recorded A P^-1 per operator precision, re-recorded after each assembly
+        self.graphs = ArnoldiGraphs(S) if graph else None
         dev = S.dev
         # solution index of every node coordinate: X(N), Y(N), Z(N), T(N) per fiber
         idx = np.empty((S.npts, 3), dtype=np.int64)
@@ -86,8 +92,14 @@ class TimeStepper:
         x0 = None
         if self.predictor is not None:
             x0 = self.predictor.predict(self)
+        fused = {}
+        if self.graphs is not None:
+            G = self.graphs
+            fused["fused_matvec"] = lambda v, out: G.get("f64")(v, out=out)
+            if self.mode == "fast":
+                fused["fused_inner"] = lambda v, out: G.get("f32c")(v, out=out)
         res = gmres(S.matvec, S.rhs, S.precond, tol=self.tol, restart=self.restart,
-                    maxiter=self.maxiter, x0=x0, inner=self.inner_operator())
+                    maxiter=self.maxiter, x0=x0, inner=self.inner_operator(), **fused)
         sol = res.x
         if self.predictor is not None:
             self.predictor.observe(self, sol)
@@ -99,6 +111,8 @@ class TimeStepper:
             # rigid motion with the solved velocities (tstep.py:1398-1406)
             S.bodies.move(S.bodies._vel(sol, 3 * S.nper), S.dt)
         S.prepare()
+        if self.graphs is not None:
+            self.graphs.reset()
         torch.cuda.synchronize()
         st = StepStats(res.iterations, res.restarts, res.residual, time.perf_counter() - t0,
                        res.guess_used)
