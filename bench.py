@@ -11,7 +11,15 @@ stresslet rates, the roofline of the dominant kernel (FP32 VALU, 28 flops per
 Oseen pair) and a CPU baseline (the oracle's C restatement of the reference's
 numba loop, OpenMP over targets, on a bounded sample of targets).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"fiber_system" (default: the large workload, 4096 fibers x 64 nodes, a
+32768-node periphery and 8 rigid spheres) reports time-steps/s to the 1e-8
+FP64 GMRES residual for: plain FP64 (the reference method on the GPU), fast
+(FP32-compensated inner matvec, FP32 MFMA fiber LU), extrapolated_fast and
+learned_fast (initial guess from the solver's own past steps).  Each mode
+runs its own trajectory from the same seeded state; untimed warm-up steps,
+including the predictor's training, precede the timed ones.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--system 0|3|4|-1]
 """
 import argparse
 import json
@@ -199,10 +207,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--system", type=int, default=-1,
-                    help="also time implicit steps of fiber-system workload 0 or 3")
-    ap.add_argument("--system-steps", type=int, default=5)
-    ap.add_argument("--train-steps", type=int, default=0)
+    ap.add_argument("--system", type=int, default=4,
+                    help="also time implicit steps of fiber-system workload 0, 3 or 4 "
+                         "(-1: raw sweep only)")
+    ap.add_argument("--system-steps", type=int, default=4)
+    ap.add_argument("--train-steps", type=int, default=8,
+                    help="untimed steps the learned predictor trains on (0: plain and fast only)")
     ap.add_argument("--lu", action="store_true", help="also time the batched fiber LU (workload 2)")
     ap.add_argument("--shell-pc", default="auto", choices=["auto", "dense", "block"])
     ap.add_argument("--no-graph", action="store_true",

@@ -1,6 +1,8 @@
 // Batched dense kernels over per-fiber blocks of uneven order (n <= 256):
-//   * LU factorisation with partial pivoting (LAPACK dgetrf semantics), FP64
-//     or equilibrated FP32, one persistent 1024-thread workgroup per block;
+//   * LU factorisation with partial pivoting (LAPACK dgetrf semantics):
+//     FP64 register/LDS-resident, one persistent 1024-thread workgroup per
+//     block; equilibrated FP32 blocked on MFMA (lu_factor_f32_mfma_kernel
+//     below), two 256-thread workgroups per CU;
 //   * forward/backward substitution with optional FP64 iterative refinement,
 //     one wavefront per block;
 //   * column-major block GEMV, one wavefront per block.
@@ -19,6 +21,9 @@
 // half-wave shuffles) -> publish pivot row / old row k / column k through LDS
 // -> every thread swaps, forms its multipliers and updates its tile.
 #include "common.h"
+
+#include <stdlib.h>
+#include <string.h>
 
 namespace f3d {
 
@@ -283,6 +288,249 @@ lu_factor_kernel(int nb, const int* __restrict__ n_arr, const long* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// FP32 blocked LU on MFMA (right-looking, panel width 32, LAPACK getrf
+// semantics on the equilibrated matrix), one 256-thread workgroup per block,
+// two workgroups per CU.  The equilibrated FP32 matrix is written to the
+// output buffer and factored in place there (L2-resident: 256 KB per block).
+// Per panel k (columns c0 = 32k .. c0 + 31, local rows 0 .. m-1 from c0 on):
+//   1. the panel is loaded into LDS (sL, column-major, one row per thread)
+//      and factored column by column with partial pivoting: block argmax
+//      (first maximum), row swap, multipliers, rank-1 update of the panel;
+//      two barriers per column;
+//   2. each thread owns one matrix column outside the panel: it applies the
+//      panel's 32 row swaps in order and, right of the panel, solves
+//      U12 = L11^-1 A12 for its column (copy kept in LDS, sU, row-major);
+//   3. A22 -= L21 U12 on 32 x 32 tiles, one tile per wave at a time:
+//      v_mfma_f32_32x32x2f32 with A = -L21 and B = U12 from LDS, C / D
+//      loaded from and stored to the factor buffer.
+// Orders that are not a multiple of 32 are padded with an identity block
+// that is never stored: virtual rows and columns stay decoupled, so the
+// factors of the real part are exactly those of the unpadded matrix.
+constexpr int kPw = 32;               // panel width = MFMA tile edge
+constexpr int kMfThreads = 256;       // 4 waves
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ void __launch_bounds__(kMfThreads, 2)
+lu_factor_f32_mfma_kernel(int nb, const int* __restrict__ n_arr, const long* __restrict__ a_off,
+                          const long* __restrict__ x_off, const int* __restrict__ order,
+                          const double* __restrict__ A, float* __restrict__ LU,
+                          double* __restrict__ rs, double* __restrict__ cs,
+                          int* __restrict__ piv, int* __restrict__ perm, int* __restrict__ info) {
+  __shared__ float sL[kPw * kLuNmax];   // panel, column c at c * 256 + local row
+  __shared__ float sU[kPw * kLuNmax];   // U12, row q at q * 256 + (col - c0 - 32)
+  __shared__ double scl_r[kLuNmax], scl_c[kLuNmax];
+  __shared__ float red_v[kMfThreads / 64];
+  __shared__ int red_i[kMfThreads / 64];
+  __shared__ int sh_piv[kPw];
+  __shared__ int sh_perm[kLuNmax];
+  __shared__ int sh_info;
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+
+  for (int w = blockIdx.x; w < nb; w += gridDim.x) {
+    const int b = order ? order[w] : w;
+    const int n = n_arr[b];
+    if (n < 1 || n > kLuNmax) {  // uniform per block
+      if (t == 0) info[b] = -1;
+      continue;
+    }
+    const int npad = (n + kPw - 1) / kPw * kPw;
+    const double* __restrict__ Ab = A + a_off[b];
+    float* __restrict__ Lb = LU + a_off[b];
+    const long xo = x_off[b];
+    __syncthreads();  // LDS reuse across blocks
+
+    // ---- power-of-two equilibration, scaled FP32 copy into Lb -------------
+    sh_perm[t] = t;
+    if (t == 0) sh_info = 0;
+    if (t < n) {
+      double mx = 0.0;
+      for (int c = 0; c < n; ++c) mx = fmax(mx, fabs(Ab[t + (long)c * n]));
+      scl_r[t] = pow2_scale(mx);
+    }
+    __syncthreads();
+    if (t < n) {
+      double mx = 0.0;
+      const double* __restrict__ col = Ab + (long)t * n;
+      for (int r = 0; r < n; ++r) mx = fmax(mx, fabs(col[r]) * scl_r[r]);
+      scl_c[t] = pow2_scale(mx);
+    }
+    __syncthreads();
+    if (t < n) {
+      const double sr = scl_r[t];
+      for (int c = 0; c < n; ++c)
+        Lb[t + (long)c * n] = (float)(Ab[t + (long)c * n] * sr * scl_c[c]);
+    }
+    __syncthreads();
+
+    for (int c0 = 0; c0 < npad; c0 += kPw) {
+      const int m = npad - c0;  // panel rows, a multiple of 32, <= 256
+      // ---- 1. panel into LDS (identity on the padding) --------------------
+      {
+        const int R = c0 + t;
+        for (int c = 0; c < kPw; ++c) {
+          const int C = c0 + c;
+          float v = 0.f;
+          if (t < m) v = (R < n && C < n) ? Lb[R + (long)C * n] : (R == C ? 1.f : 0.f);
+          sL[c * kLuNmax + t] = v;
+        }
+      }
+      __syncthreads();
+      for (int j = 0; j < kPw; ++j) {
+        // argmax |a(r, j)| over local rows j <= r < m, first maximum wins
+        // (barrier B1 below also orders this step's swap after every thread's
+        // last update of its own row; B2 orders the update after the swap)
+        float best = (t >= j && t < m) ? fabsf(sL[j * kLuNmax + t]) : -1.f;
+        int bi = t;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+          const float ob = __shfl_xor(best, o, 64);
+          const int oi = __shfl_xor(bi, o, 64);
+          if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+          }
+        }
+        if (lane == 0) {
+          red_v[wave] = best;
+          red_i[wave] = bi;
+        }
+        __syncthreads();  // B1
+        float pb = red_v[0];
+        int p = red_i[0];
+#pragma unroll
+        for (int q = 1; q < kMfThreads / 64; ++q) {
+          const float vq = red_v[q];
+          if (vq > pb || (vq == pb && red_i[q] < p)) {
+            pb = vq;
+            p = red_i[q];
+          }
+        }
+        if (!(pb >= 0.f) || p < j || p >= m) p = j;  // NaN column: keep the diagonal
+        // row swap, threads 0..31 one panel column each
+        if (t < kPw && p != j) {
+          const float x = sL[t * kLuNmax + j];
+          sL[t * kLuNmax + j] = sL[t * kLuNmax + p];
+          sL[t * kLuNmax + p] = x;
+        }
+        if (t == 0) sh_piv[j] = p;
+        __syncthreads();  // B2
+        const float pivot = sL[j * kLuNmax + j];
+        if (t == 0 && pivot == 0.f && sh_info == 0 && c0 + j < n) sh_info = c0 + j + 1;
+        if (t > j && t < m) {
+          const float inv = pivot != 0.f ? 1.f / pivot : 0.f;
+          const float l = sL[j * kLuNmax + t] * inv;
+          sL[j * kLuNmax + t] = l;
+          for (int c = j + 1; c < kPw; ++c)
+            sL[c * kLuNmax + t] = fmaf(-l, sL[c * kLuNmax + j], sL[c * kLuNmax + t]);
+        }
+      }
+      __syncthreads();
+      // panel back to the factor buffer; pivots and the row permutation
+      {
+        const int R = c0 + t;
+        if (t < m && R < n)
+          for (int c = 0; c < kPw && c0 + c < n; ++c) Lb[R + (long)(c0 + c) * n] = sL[c * kLuNmax + t];
+      }
+      if (t == 0) {
+        for (int j = 0; j < kPw; ++j) {
+          const int p = sh_piv[j];
+          if (c0 + j < n) piv[xo + c0 + j] = c0 + p;
+          const int tmp = sh_perm[c0 + j];
+          sh_perm[c0 + j] = sh_perm[c0 + p];
+          sh_perm[c0 + p] = tmp;
+        }
+      }
+      // ---- 2. row swaps outside the panel, TRSM right of it ---------------
+      if (t < npad && (t < c0 || t >= c0 + kPw)) {
+        float* __restrict__ colp = Lb + (long)t * n;
+        if (t < n) {
+          for (int j = 0; j < kPw; ++j) {
+            const int p = sh_piv[j];
+            if (p != j && c0 + p < n) {
+              const float x = colp[c0 + j];
+              colp[c0 + j] = colp[c0 + p];
+              colp[c0 + p] = x;
+            }
+          }
+        }
+        if (t >= c0 + kPw) {
+          const int nv = min(kPw, n - c0);  // real rows of the panel (uniform, >= 1)
+          const bool full = t < n && nv == kPw;
+          float x[kPw];
+          if (full) {
+#pragma unroll
+            for (int i = 0; i < kPw; ++i) x[i] = colp[c0 + i];
+          } else {
+            const bool live = t < n;
+#pragma unroll
+            for (int i = 0; i < kPw; ++i) x[i] = (live && i < nv) ? colp[c0 + i] : 0.f;
+          }
+          // column-oriented forward substitution with the unit-lower L11
+#pragma unroll
+          for (int q = 0; q < kPw - 1; ++q) {
+            const float xq = x[q];
+#pragma unroll
+            for (int i = q + 1; i < kPw; ++i) x[i] = fmaf(-sL[q * kLuNmax + i], xq, x[i]);
+          }
+          const int cu = t - c0 - kPw;
+#pragma unroll
+          for (int i = 0; i < kPw; ++i) sU[i * kLuNmax + cu] = x[i];
+          if (full) {
+#pragma unroll
+            for (int i = 0; i < kPw; ++i) colp[c0 + i] = x[i];
+          } else if (t < n) {
+#pragma unroll
+            for (int i = 0; i < kPw; ++i)
+              if (i < nv) colp[c0 + i] = x[i];
+          }
+        }
+      }
+      __syncthreads();
+      // ---- 3. trailing update on MFMA ---------------------------------------
+      const int nr = (npad - c0 - kPw) / kPw;  // trailing tiles per dimension
+      for (int tile = wave; tile < nr * nr; tile += kMfThreads / 64) {
+        const int ti = tile / nr, tj = tile % nr;
+        const int R0 = c0 + kPw + kPw * ti, C0 = c0 + kPw + kPw * tj;
+        const int h = lane >> 5, cl = lane & 31;
+        const int C = C0 + cl;
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int R = R0 + 8 * (r >> 2) + 4 * h + (r & 3);
+          acc[r] = (R < n && C < n) ? Lb[R + (long)C * n] : 0.f;
+        }
+        // A (32 x 2 per step): lane holds L21(R0 + cl, 2s + h);
+        // B (2 x 32 per step): lane holds U12(2s + h, C0 + cl)
+        const float* __restrict__ la = sL + (R0 - c0) + cl;
+        const float* __restrict__ ub = sU + (C0 - c0 - kPw) + cl;
+#pragma unroll
+        for (int s = 0; s < kPw / 2; ++s) {
+          const int q = 2 * s + h;
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(-la[q * kLuNmax], ub[q * kLuNmax], acc, 0,
+                                                     0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int R = R0 + 8 * (r >> 2) + 4 * h + (r & 3);
+          if (R < n && C < n) Lb[R + (long)C * n] = acc[r];
+        }
+      }
+      __syncthreads();
+    }
+
+    if (t < n) {
+      perm[xo + t] = sh_perm[t];
+      rs[xo + t] = scl_r[t];
+      cs[xo + t] = scl_c[t];
+    }
+    if (t == 0) info[b] = sh_info;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // One wavefront per block: lane owns rows lane + 64 m, m < 4.
 constexpr int kSlots = 4;
 
@@ -538,9 +786,19 @@ extern "C" int f3d_lu_factor_batched(int precision, int64_t nb, int32_t nmax, co
                (double*)nullptr, piv, perm, info, 0);
   } else {
     F3D_REQUIRE(lu32 && rs && cs, F3D_EINVAL, "lu_factor: lu32/rs/cs is NULL");
-    F3D_LAUNCH((lu_factor_kernel<float, 0>), dim3(grid), dim3(1024), 0, st, (int)nb, n,
-               (const long*)a_off, (const long*)x_off, order, A, lu32, rs, cs, piv, perm, info,
-               1);
+    // F3D_LU32_KERNEL=vector selects the register-resident VALU factorisation
+    const char* sel = getenv("F3D_LU32_KERNEL");
+    if (sel && strcmp(sel, "vector") == 0) {
+      F3D_LAUNCH((lu_factor_kernel<float, 0>), dim3(grid), dim3(1024), 0, st, (int)nb, n,
+                 (const long*)a_off, (const long*)x_off, order, A, lu32, rs, cs, piv, perm, info,
+                 1);
+    } else {
+      const long g2 = 2l * num_cus();
+      const unsigned grid2 = (unsigned)(nb < g2 ? nb : g2);
+      F3D_LAUNCH(lu_factor_f32_mfma_kernel, dim3(grid2), dim3(kMfThreads), 0, st, (int)nb, n,
+                 (const long*)a_off, (const long*)x_off, order, A, lu32, rs, cs, piv, perm,
+                 info);
+    }
   }
   return F3D_OK;
 }

@@ -123,3 +123,38 @@ def test_batched_lu_on_real_fiber_blocks_config2_shape():
         assert np.linalg.norm(r[o:o + n]) <= 1e-12 * np.linalg.norm(bh[o:o + n]), k
     # forward error stays at the conditioning level of the FP64 factorisation
     assert e32 < max(1e4 * e64, 1e-9)
+
+
+@pytest.mark.parametrize("kernel", ["mfma", "vector"])
+def test_lu_factor_f32_is_a_factorisation(kernel, monkeypatch):
+    """The equilibrated FP32 factors satisfy P (Dr A Dc) = L U to FP32
+    accuracy, with ipiv and perm consistent (blocked MFMA kernel and the
+    register-resident VALU kernel; orders not divisible by 32 included)."""
+    print(f"case lu f32 factor {kernel}", flush=True)
+    if kernel == "vector":
+        monkeypatch.setenv("F3D_LU32_KERNEL", "vector")
+    else:
+        monkeypatch.delenv("F3D_LU32_KERNEL", raising=False)
+    mats = _mats(SIZES, 7)
+    bs = BlockSet(SIZES)
+    lu = BatchedLU(bs, bs.pack(mats), precision="f32c")
+    torch.cuda.synchronize()
+    assert lu.info.cpu().numpy().tolist() == [0] * len(SIZES)
+    LU = bs.unpack(lu.lu32.double())
+    piv, perm = lu.piv.cpu().numpy(), lu.perm.cpu().numpy()
+    rs, cs = lu.rs.cpu().numpy(), lu.cs.cpu().numpy()
+    for b, M in enumerate(mats):
+        n, o = SIZES[b], bs.x_off_h[b]
+        As = rs[o:o + n, None] * M * cs[None, o:o + n]
+        p = perm[o:o + n]
+        ref = np.arange(n)
+        for j in range(n):
+            q = piv[o + j]
+            assert j <= q < n
+            ref[[j, q]] = ref[[q, j]]
+        np.testing.assert_array_equal(p, ref)
+        L = np.tril(LU[b], -1) + np.eye(n)
+        U = np.triu(LU[b])
+        assert np.abs(L).max() <= 1.0 + 1e-6          # partial pivoting
+        err = np.abs(As[p] - L @ U).max() / np.abs(As).max()
+        assert err < 2e-5, (n, err)
