@@ -1,0 +1,99 @@
+"""GMRES host logic on CPU tensors: convergence against a direct solve,
+restarts, the inexact inner operator and its accuracy-floor switch, initial
+guesses (accepted, rejected) and the fused A P^-1 path."""
+import numpy as np
+import torch
+from scipy.linalg import block_diag
+
+from fiber3d_amd.gmres import gmres
+
+
+def _system(n=160, seed=0):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(n, n)) / np.sqrt(n) + 3.0 * np.eye(n)
+    b = rng.normal(size=n)
+    At = torch.from_numpy(A)
+    mv = lambda v, out=None: torch.mv(At, v, out=out) if out is not None else torch.mv(At, v)
+    # block-diagonal (4x4) preconditioner: exact inverse of A's diagonal blocks
+    Pm = torch.from_numpy(block_diag(*[np.linalg.inv(A[i:i + 4, i:i + 4])
+                                       for i in range(0, n, 4)]))
+
+    def pc(v, out=None):
+        y = torch.mv(Pm, v)
+        if out is None:
+            return y
+        out.copy_(y)
+        return out
+    return A, b, mv, pc, torch.from_numpy(b)
+
+
+def test_converges_to_direct_solve():
+    A, b, mv, pc, bt = _system()
+    res = gmres(mv, bt, pc, tol=1e-12)
+    assert res.info == 0 and res.residual <= 1e-12
+    x = np.linalg.solve(A, b)
+    assert np.linalg.norm(res.x.numpy() - x) / np.linalg.norm(x) < 1e-10
+    # true residual recomputed independently
+    assert np.linalg.norm(b - A @ res.x.numpy()) / np.linalg.norm(b) <= 1.01e-12
+
+
+def test_restarts_still_converge():
+    A, b, mv, pc, bt = _system(seed=1)
+    res = gmres(mv, bt, pc, tol=1e-10, restart=3, maxiter=200)
+    assert res.info == 0 and res.restarts > 1
+    assert np.linalg.norm(b - A @ res.x.numpy()) / np.linalg.norm(b) <= 1.01e-10
+
+
+def test_inexact_inner_operator_reaches_fp64_tolerance():
+    """An inner operator with 1e-7 relative error cannot by itself bring the
+    true residual to 1e-10; GMRES detects the floor and finishes with the
+    exact operator."""
+    A, b, mv, pc, bt = _system(seed=2)
+    rng = np.random.default_rng(9)
+    E = torch.from_numpy(rng.normal(size=A.shape) * 1e-7 * np.abs(A).max())
+    Ai = torch.from_numpy(A) + E
+
+    def inner(v, out=None):
+        y = torch.mv(Ai, v)
+        if out is None:
+            return y
+        out.copy_(y)
+        return out
+    res = gmres(mv, bt, pc, tol=1e-10, inner=inner)
+    assert res.info == 0 and res.residual <= 1e-10
+    assert res.inner_cycles >= 1 and res.restarts > res.inner_cycles
+    assert np.linalg.norm(b - A @ res.x.numpy()) / np.linalg.norm(b) <= 1.01e-10
+
+
+def test_initial_guess_accepted_and_rejected():
+    A, b, mv, pc, bt = _system(seed=3)
+    x = np.linalg.solve(A, b)
+    cold = gmres(mv, bt, pc, tol=1e-10)
+    warm = gmres(mv, bt, pc, tol=1e-10, x0=torch.from_numpy(x * (1 + 1e-6)))
+    assert warm.guess_used and warm.iterations < cold.iterations
+    bad = gmres(mv, bt, pc, tol=1e-10, x0=torch.full_like(bt, float("nan")))
+    assert not bad.guess_used and bad.info == 0
+    far = gmres(mv, bt, pc, tol=1e-10, x0=torch.from_numpy(-100.0 * x))
+    assert not far.guess_used and far.info == 0      # residual above ||b||: zero guess
+    assert np.allclose(bad.x.numpy(), cold.x.numpy(), rtol=0, atol=1e-12)
+
+
+def test_fused_operator_path_matches_separate_calls():
+    A, b, mv, pc, bt = _system(seed=4)
+    calls = {"n": 0}
+
+    def fused(v, out):
+        calls["n"] += 1
+        out.copy_(mv(pc(v)))
+        return out
+    r1 = gmres(mv, bt, pc, tol=1e-10)
+    r2 = gmres(mv, bt, pc, tol=1e-10, fused_matvec=fused)
+    assert calls["n"] == r2.iterations
+    assert r1.iterations == r2.iterations
+    assert np.abs(r1.x.numpy() - r2.x.numpy()).max() < 1e-13
+
+
+def test_zero_rhs():
+    A, b, mv, pc, bt = _system(seed=5)
+    res = gmres(mv, torch.zeros_like(bt), pc, tol=1e-10)
+    assert res.info == 0 and res.iterations == 0 and float(res.x.abs().max()) == 0.0
