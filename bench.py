@@ -38,6 +38,8 @@ OSEEN_FLOPS = 28      # per pair: 3 sub, r^2 5, rsqrt 1, d.f 5, r^-2 1, scale 1,
 STRESSLET_FLOPS = 30
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak (packed)
 PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0
 # HBM bytes per launch of the FP32-compensated Oseen pair kernel at N = M = 65536:
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --steps 2
 # --warmup 1 --no-cpu` (17454.4 and 49152 KiB per dispatch), scaled by what the
@@ -155,8 +157,21 @@ def lu_leg(reps=5, cpu=True):
     xs = torch.from_numpy(rng.normal(size=S.bs.nrows)).cuda()
     b = torch.empty_like(xs)
     S.gA(S.A_buf, xs, b)
+    nn = float(np.sum(n**2))
     out = {"workload": f"{len(fibers)} fibers, N in {cfg.fibers.num_points}, n = 4N",
            "factor_flops": flops}
+    # A_sq x for every block (block_gemv_kernel): HBM-bound, 8 n^2 bytes per block
+    S.gA(S.A_buf, xs, b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        S.gA(S.A_buf, xs, b)
+    torch.cuda.synchronize()
+    tg = (time.perf_counter() - t0) / reps
+    out["block_gemv"] = {"ms": round(tg * 1e3, 4), "bytes": 8 * nn,
+                         "roofline": {"bound": "hbm", "achieved": round(8 * nn / tg / 1e9, 1),
+                                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                      "frac": round(8 * nn / tg / 1e9 / PEAK_HBM_GBS, 3)}}
     for prec in ("f64", "f32c"):
         lu = BatchedLU(S.bs, S.A_buf, precision=prec, refine=10, tol=1e-12)
         torch.cuda.synchronize()
@@ -173,9 +188,22 @@ def lu_leg(reps=5, cpu=True):
         torch.cuda.synchronize()
         ts = (time.perf_counter() - t0) / reps
         err = float(torch.linalg.norm(y - xs) / torch.linalg.norm(xs))
+        sweeps = int(lu.iters.max()) if prec == "f32c" else 0
+        # solve bytes: the factor once per substitution, plus A (FP64) per
+        # refinement sweep; sweeps counted at the block maximum
+        fb = 8 if prec == "f64" else 4
+        sbytes = nn * (fb * (1 + sweeps) + 8 * sweeps)
+        peak = PEAK_FP64_TFLOPS if prec == "f64" else PEAK_FP32_TFLOPS
         out[prec] = {"factor_ms": round(tf * 1e3, 3), "factor_gflops": round(flops / tf / 1e9, 1),
+                     "factor_roofline": {"bound": "valu" if prec == "f64" else "mfma",
+                                         "achieved": round(flops / tf / 1e12, 3), "peak": peak,
+                                         "unit": "TFLOP/s",
+                                         "frac": round(flops / tf / 1e12 / peak, 4)},
                      "solve_ms": round(ts * 1e3, 3), "forward_error": err,
-                     "max_refine_sweeps": int(lu.iters.max()) if prec == "f32c" else 0}
+                     "max_refine_sweeps": sweeps,
+                     "solve_roofline": {"bound": "hbm", "achieved": round(sbytes / ts / 1e9, 1),
+                                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                        "frac": round(sbytes / ts / 1e9 / PEAK_HBM_GBS, 3)}}
     if cpu:
         import scipy.linalg as sla
         blocks = S.bs.unpack(S.A_buf)[:256]
@@ -188,6 +216,34 @@ def lu_leg(reps=5, cpu=True):
                                "sample": "first 256 of the fibers", "factor_gflops":
                                round(float(np.sum(2 / 3 * nb**3)) / dt / 1e9, 2)}
     return out
+
+
+def mlp_leg(reps=20):
+    """The learned guess's hidden layer at the large workload: one BF16 MFMA
+    GEMM with bias + ReLU epilogue over every fiber node, (262144 x 256) x
+    (256 x 256), FP32 accumulate (f3d_gemm_bf16)."""
+    import torch
+    from fiber3d_amd import learned
+    M, K, N = 262144, 256, 256
+    g = torch.Generator().manual_seed(0)
+    A = torch.randn(M, K, generator=g).cuda()
+    B = (torch.randn(K, N, generator=g) / 16).cuda()
+    bias = torch.randn(N, generator=g).cuda()
+    C = torch.empty(M, N, device="cuda")
+    learned.gemm(A, B, C, epi=learned.EPI_BIAS_RELU, bias=bias)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        learned.gemm(A, B, C, epi=learned.EPI_BIAS_RELU, bias=bias)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    ref = torch.relu(A.to(torch.bfloat16).float() @ B.to(torch.bfloat16).float() + bias)
+    err = float(torch.linalg.norm(C - ref) / torch.linalg.norm(ref))
+    fl = 2.0 * M * N * K
+    return {"shape": [M, K, N], "ms": round(dt * 1e3, 4), "rel_err_vs_fp32_of_bf16_inputs": err,
+            "roofline": {"bound": "mfma", "achieved": round(fl / dt / 1e12, 1),
+                         "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(fl / dt / 1e12 / PEAK_BF16_TFLOPS, 4)}}
 
 
 def copy_probe(mib):
@@ -213,7 +269,8 @@ def main():
     ap.add_argument("--system-steps", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=8,
                     help="untimed steps the learned predictor trains on (0: plain and fast only)")
-    ap.add_argument("--lu", action="store_true", help="also time the batched fiber LU (workload 2)")
+    ap.add_argument("--no-lu", dest="lu", action="store_false",
+                    help="skip the batched fiber LU (workload 2) and MLP GEMM legs")
     ap.add_argument("--shell-pc", default="auto", choices=["auto", "dense", "block"])
     ap.add_argument("--no-graph", action="store_true",
                     help="run the Arnoldi operator eagerly instead of as a recorded HIP graph")
@@ -308,6 +365,7 @@ def main():
     }
     if args.lu:
         line["batched_lu"] = lu_leg(cpu=not args.no_cpu)
+        line["mlp_gemm_bf16"] = mlp_leg()
     if args.system >= 0:
         line["fiber_system"] = system_leg(args.system, args.system_steps, args.train_steps,
                                           shell_pc=args.shell_pc, graph=not args.no_graph)
