@@ -60,6 +60,9 @@ SIGNATURES = {
     "f3d_gemm_bf16": (ctypes.c_int, [
         ctypes.c_int, _i64, _i64, _i64, _i32, _c_p, _i64, _i64, _c_p, _i64, _i64,
         _c_p, _i64, _i64, _c_p, _c_p, _c_p, ctypes.c_float, _c_p]),
+    "f3d_fiber_assemble": (ctypes.c_int, [
+        _i64, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
+        _f64, _f64, _f64, _f64, _f64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "f3d_block_gemv_batched": (ctypes.c_int, [
         _i64, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _f64, _f64, _c_p]),
 }

@@ -49,12 +49,23 @@ def _ensure_tuned():
         autotune.apply()
 
 
+# Workspace buffers replaced by a larger one stay allocated for the life of
+# the process: a recorded HIP graph keeps the device address it was recorded
+# with, and freeing the buffer would hand that memory to other tensors.
+_ws_retired = []
+
+
 def workspace(nbytes, tag="default"):
-    """A cached device byte buffer of at least nbytes (grown on demand)."""
+    """A cached device byte buffer of at least nbytes (grown by >= 1.5x on
+    demand; a replaced buffer is retired, never freed)."""
     key = (tag, torch.cuda.current_device())
     buf = _ws.get(key)
     if buf is None or buf.numel() < nbytes:
-        buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())
+        size = max(int(nbytes), 256)
+        if buf is not None:
+            _ws_retired.append(buf)
+            size = max(size, int(1.5 * buf.numel()))
+        buf = torch.empty(size, dtype=torch.uint8, device=device())
         _ws[key] = buf
     return buf
 

@@ -316,7 +316,8 @@ class FiberSystem:
     def __init__(self, fibers, shell=None, eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
                  lengths=None, pair_precision="f64", lu_precision="f64", lu_refine=6,
-                 lu_tol=1e-13, reg=5e-3, eps=1e-5, bodies=None, body_force=None):
+                 lu_tol=1e-13, reg=5e-3, eps=1e-5, bodies=None, body_force=None,
+                 assembly="torch"):
         self.dev = device()
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = _c0c1(epsilon, eta)
@@ -324,6 +325,9 @@ class FiberSystem:
         self.reg, self.eps = reg, eps
         self.end_force = end_force
         self.pair_precision = pair_precision
+        if assembly not in ("torch", "hip"):
+            raise ValueError("assembly is 'torch' or 'hip'")
+        self.assembly = assembly
         self.lu_precision, self.lu_refine, self.lu_tol = lu_precision, lu_refine, lu_tol
         sizes = np.array([np.asarray(x).shape[0] for x in fibers], dtype=np.int64)
         if sizes.min() < 5:
@@ -377,6 +381,20 @@ class FiberSystem:
         # tension projection: per component c, (4N-14) x N blocks
         self.gQ = [BlockGemv(4 * sizes - 14, sizes, c * self.npts + fo, 4 * fo) for c in range(3)]
         self.gA = BlockGemv(4 * sizes, 4 * sizes, 4 * fo, 4 * fo)
+        # launch tables of the fused assembly kernel, per node count
+        ti = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int64)).to(self.dev)
+        f_off_all = self.gF.m_off_h.reshape(self.nfib, 3)[:, 0]
+        for N, bk in self.buckets.items():
+            ids = bk["ids"]
+            P = bk["P"]
+            bk["asm"] = {
+                "node0": ti(self.offs[ids]), "a_off": ti(self.bs.a_off_h[ids]),
+                "f_off": ti(f_off_all[ids]),
+                "q_off": ti(np.stack([self.gQ[c].m_off_h[ids] for c in range(3)], axis=1)),
+                "D0": torch.stack(bk["D0"]).contiguous(),
+                "PX": P[:N - 4, :N].contiguous(), "PT": P[3 * (N - 4):, 3 * N:].contiguous(),
+                "length": self.lengths[bk["ids_t"]].contiguous(),
+            }
         # work buffers
         z = lambda n: torch.zeros(n, dtype=torch.float64, device=self.dev)
         self.f_soa = z(3 * self.npts)
@@ -401,7 +419,12 @@ class FiberSystem:
         self.xs = torch.empty((self.npts, 3), dtype=torch.float64, device=dev)
         self._rhs_parts = {}
         self._ctx = {}
-        for N, bk in self.buckets.items():
+        if self.assembly == "hip":
+            self._assemble_hip(A_buf, F_buf, Q_buf)
+            buckets = {}
+        else:
+            buckets = self.buckets
+        for N, bk in buckets.items():
             ids_t = bk["ids_t"]
             B = ids_t.numel()
             node = bk["start"][:, None] + torch.arange(N, device=dev)[None, :]  # (B, N)
@@ -469,6 +492,26 @@ class FiberSystem:
         self._build_rhs()
         self.lu = BatchedLU(self.bs, self.A_buf, precision=self.lu_precision,
                             refine=self.lu_refine, tol=self.lu_tol)
+
+    def _assemble_hip(self, A_buf, F_buf, Q_buf):
+        """A_sq, force rows, tension projections, tangents and weights of every
+        fiber from f3d_fiber_assemble, one launch per node count."""
+        self.xss = torch.empty((self.npts, 3), dtype=torch.float64, device=self.dev)
+        lib = L.load()
+        for N, bk in self.buckets.items():
+            t = bk["asm"]
+            B = len(bk["ids"])
+            rc = lib.f3d_fiber_assemble(
+                B, N, L.ptr(t["node0"]), L.ptr(t["length"]), L.ptr(t["a_off"]), L.ptr(t["f_off"]),
+                L.ptr(t["q_off"]), L.ptr(self.X), L.ptr(t["D0"]), L.ptr(t["PX"]), L.ptr(t["PT"]),
+                L.ptr(bk["w0"]), float(self.E), float(self.dt), float(self.c0), float(self.c1),
+                float(self.penalty), L.ptr(A_buf), L.ptr(F_buf), L.ptr(Q_buf[0]), L.ptr(Q_buf[1]),
+                L.ptr(Q_buf[2]), L.ptr(self.xs), L.ptr(self.xss), L.ptr(self.weights), _stream())
+            L.check(rc, "f3d_fiber_assemble")
+            node = bk["start"][:, None] + torch.arange(N, device=self.dev)[None, :]
+            sc = (2.0 / self.lengths[bk["ids_t"]])[:, None, None]
+            self._ctx[N] = (node, self.X[node], self.xs[node], self.xss[node],
+                            bk["D0"][0][None] * sc, bk["P"], B)
 
     def _build_rhs(self):
         """RHS: explicit flows of the plus-end forces and per-fiber RHS

@@ -196,6 +196,30 @@ int f3d_lu_solve_batched(int precision, int64_t nb, int32_t nmax, const int32_t*
                          int32_t refine, double tol, int32_t* iters, void* stream);
 
 /*
+ * Per-fiber assembly of the implicit fiber system for nfib fibers of one node
+ * count N (5 <= N <= 64), FP64, one workgroup per fiber.  Fiber f starts at
+ * node node0[f] of the AoS positions X (npts x 3) and has length length[f].
+ * D0: the four derivative matrices of the unit parameter interval, (4, N, N)
+ * row-major; PX ((N-4) x N) and PT ((N-2) x N) the projections onto the
+ * interior collocation points; w0 (N) the unit quadrature weights.  Writes:
+ *   A_out + a_off[f]: A_sq (4N x 4N, column-major): the projected linear
+ *     operator plus the 14 force- and torque-free end conditions;
+ *   F_out + f_off[f]: the three force-operator row blocks (N x 4N each,
+ *     column-major, adjacent);
+ *   Q0/Q1/Q2 + q_off[3f + c]: the tension projections ((4N-14) x N);
+ *   xs_out, xss_out (npts x 3) and w_out (npts) at the fiber's nodes.
+ * Replaces fiber.form_linear_operator (python/fiber/fiber.py:322-447),
+ * apply_BC_rectangular (:810-961), force_operator (:1399-1431) and the
+ * per-fiber tension projection of python/tstep/tstep_utils.py:1238-1336.
+ */
+int f3d_fiber_assemble(int64_t nfib, int32_t N, const int64_t* node0, const double* length,
+                       const int64_t* a_off, const int64_t* f_off, const int64_t* q_off,
+                       const double* X, const double* D0, const double* PX, const double* PT,
+                       const double* w0, double E, double dt, double c0, double c1,
+                       double penalty, double* A_out, double* F_out, double* Q0, double* Q1,
+                       double* Q2, double* xs_out, double* xss_out, double* w_out, void* stream);
+
+/*
  * y_b = alpha * M_b x_b + beta * y_b for a batch of dense COLUMN-major blocks
  * of shape m[b] x k[b] (leading dimension m[b]) stored at M + m_off[b]; x at
  * x_off[b], y at y_off[b] (y must not alias x).  m[b] <= mmax <= 1024.
