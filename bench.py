@@ -70,7 +70,7 @@ def cpu_baseline(x, f, n_targets=4096, cores=None):
 
 
 def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_bodies=True,
-               graph=True):
+               graph=False):
     """time-steps/s of the implicit fiber step, plain FP64 vs fast (+ learned guess).
 
     Each mode runs its own trajectory from the same seeded initial state:
@@ -272,8 +272,8 @@ def main():
     ap.add_argument("--no-lu", dest="lu", action="store_false",
                     help="skip the batched fiber LU (workload 2) and MLP GEMM legs")
     ap.add_argument("--shell-pc", default="auto", choices=["auto", "dense", "block"])
-    ap.add_argument("--no-graph", action="store_true",
-                    help="run the Arnoldi operator eagerly instead of as a recorded HIP graph")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the Arnoldi operator as a recorded HIP graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (calibrates the HBM "
                          "byte counters FETCH_SIZE / WRITE_SIZE against a known size)")
@@ -368,7 +368,7 @@ def main():
         line["mlp_gemm_bf16"] = mlp_leg()
     if args.system >= 0:
         line["fiber_system"] = system_leg(args.system, args.system_steps, args.train_steps,
-                                          shell_pc=args.shell_pc, graph=not args.no_graph)
+                                          shell_pc=args.shell_pc, graph=args.graph)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(x, f)
     print(json.dumps(line), flush=True)

@@ -28,7 +28,7 @@ import torch
 
 from . import _lib as L
 from . import fiber_matrices as FMX
-from .dense import BatchedLU, BlockGemv, BlockSet
+from .dense import BatchedLU, BlockGemv, BlockSet, DenseRowBlocks
 from .kernels import _stream, device, pairsum, periphery_apply
 
 
@@ -70,10 +70,10 @@ class Shell:
 
     pc='dense' builds the reference's dense operator (tstep.py:328-345) and
     inverts it once; the reference LU-factors it (tstep.py:346-351) and
-    applies lu_solve, the same linear map.  The inverse is applied as one
-    GEMV, which a HIP graph can capture (rocSOLVER's getrs allocates) and which
-    is bandwidth-bound instead of a latency-bound pair of triangular sweeps.
-    pc_f32=True also keeps an FP32 copy for the fast mode.  pc='block'
+    applies lu_solve, the same linear map.  The inverse is applied in FP64 by
+    the libf3d block GEMV: bandwidth-bound instead of a latency-bound pair of
+    triangular sweeps, and free of BLAS/rocSOLVER calls, whose workspaces a
+    HIP graph cannot keep.  pc_f32 is accepted and ignored.  pc='block'
     inverts only each node's 3x3 diagonal block, -E_i/w_i - I/w_i + n_i n_i^T
     (0.1 ms per apply at 32768 nodes; about twice the GMRES iterations of the
     dense form).  The dense inverse grows as (3N)^2: 4.6 GB at 8000 nodes,
@@ -99,11 +99,11 @@ class Shell:
         if pc == "auto":
             pc = "dense" if self.n <= self.DENSE_MAX else "block"
         self.pc = pc
-        self.minv = self.minv32 = None
+        self.minv = None
         if pc == "dense":
-            self.minv = dense_inverse(self.dense_matrix())
-            if pc_f32:
-                self.minv32 = self.minv.float()
+            # FP64 inverse applied with the batched block GEMV (no BLAS call,
+            # so the preconditioner can be recorded in a HIP graph)
+            self.minv = DenseRowBlocks(dense_inverse(self.dense_matrix()))
         elif pc == "block":
             E = torch.stack(self.e, dim=2)  # E[i, a, c] = e_c[i, a]
             winv = (1.0 / self.w)[:, None, None]
@@ -138,12 +138,16 @@ class Shell:
                                precision=precision, out=out)
 
     def precond(self, r, out, f32=False):
+        """Shell block of P^-1 (f32 is accepted for symmetry with the fiber
+        factors; the shell inverse is applied in FP64, an HBM-bound GEMV)."""
         if self.pc == "block":
-            out.view(self.n, 3).copy_(torch.matmul(self.Dinv, r.view(self.n, 3, 1))[..., 0])
-        elif f32 and self.minv32 is not None:
-            out.copy_(torch.mv(self.minv32, r.float()))
+            # per node: out_i = D_i^-1 r_i, written as elementwise products
+            D, rv = self.Dinv, r.view(self.n, 3)
+            out.view(self.n, 3).copy_(D[:, :, 0] * rv[:, 0:1] + D[:, :, 1] * rv[:, 1:2]
+                                      + D[:, :, 2] * rv[:, 2:3])
         else:
-            torch.mv(self.minv, r, out=out)
+            src = r if (r.is_contiguous() and r.data_ptr() != out.data_ptr()) else r.clone()
+            self.minv(src, out)
         return out
 
 
@@ -183,11 +187,10 @@ class Bodies:
         # rotation of every body from the pose its inverse block was built in
         self.rot = torch.eye(3, dtype=torch.float64, device=dev).repeat(self.count, 1, 1)
         self.update_geometry()
-        self.ainv = self._invert(self.pc_blocks())
-
-    @staticmethod
-    def _invert(A):
-        return torch.stack([dense_inverse(A[k]) for k in range(A.shape[0])])
+        self.ainv = DenseRowBlocks(torch.stack([dense_inverse(A) for A in self.pc_blocks()]))
+        m = 3 * self.nb + 6
+        self._xb = torch.empty(self.count * m, dtype=torch.float64, device=dev)
+        self._yb = torch.empty(self.count * m, dtype=torch.float64, device=dev)
 
     def update_geometry(self):
         """Centre-relative nodes and subtraction vectors for the current pose."""
@@ -283,11 +286,19 @@ class Bodies:
         ob[:, 3 * self.nb + 3:] = vel[:, 3:] - T
 
     def precond(self, x, out, start):
-        xb = self.block(x, start).view(self.count, self.nb + 2, 3)
-        # rows are 3-vectors v^T: (R^T v)^T = v^T R, and back (R y)^T = y^T R^T
-        yb = torch.bmm(self.ainv, torch.bmm(xb, self.rot).reshape(self.count, -1, 1))
-        yb = torch.bmm(yb.view(self.count, self.nb + 2, 3), self.rot.transpose(1, 2).contiguous())
-        self.block(out, start).copy_(yb.reshape(self.count, -1))
+        """Rotate every 3-vector of a body's slice into the body frame
+        ((R^T v)_c = sum_a R_ac v_a), apply A0^-1 (batched block GEMV),
+        rotate back ((R y)_c = sum_a R_ca y_a).  Elementwise products and the
+        libf3d GEMV only, no BLAS, so the step can be recorded in a graph."""
+        k = self.nb + 2
+        R = self.rot[:, None]                              # (count, 1, 3, 3)
+        v = self.block(x, start).view(self.count, k, 3)
+        xb = self._xb.view(self.count, k, 3)
+        xb.copy_(v[..., 0:1] * R[..., 0, :] + v[..., 1:2] * R[..., 1, :] + v[..., 2:3] * R[..., 2, :])
+        self.ainv(self._xb, self._yb)
+        y = self._yb.view(self.count, k, 3)
+        self.block(out, start).view(self.count, k, 3).copy_(
+            y[..., 0:1] * R[..., :, 0] + y[..., 1:2] * R[..., :, 1] + y[..., 2:3] * R[..., :, 2])
 
     def move(self, vel, dt):
         """Rigid update: centre += U dt, nodes rotated by Omega dt (Rodrigues)."""
