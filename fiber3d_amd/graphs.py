@@ -20,9 +20,14 @@ import torch
 
 
 class CapturedOperator:
-    def __init__(self, fn, n, device, dtype=torch.float64, warmup=2):
+    def __init__(self, fn, n, device, dtype=torch.float64, warmup=2, keep=()):
         """fn(x, out) computes out = op(x) for (n,) tensors; it is recorded
-        once, on static buffers self.x -> self.y."""
+        once, on static buffers self.x -> self.y.  `keep` lists tensors the
+        recording addresses that nothing else keeps alive (scratch buffers
+        of fn): the graph writes them on every replay, so freeing one would
+        let the allocator hand its memory to another tensor."""
+        self.fn = fn
+        self.keep = tuple(keep)
         self.x = torch.zeros(n, dtype=dtype, device=device)
         self.y = torch.zeros(n, dtype=dtype, device=device)
         side = torch.cuda.Stream(device=device)
@@ -69,7 +74,7 @@ class ArnoldiGraphs:
                 S.precond(x, out=z)
                 S.matvec(z, out=out, precision=precision)
 
-            op = CapturedOperator(fn, S.size, S.dev)
+            op = CapturedOperator(fn, S.size, S.dev, keep=(z,))
             self._ops[precision] = op
             self.captures += 1
         return op

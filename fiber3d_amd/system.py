@@ -195,8 +195,12 @@ class Bodies:
     def update_geometry(self):
         """Centre-relative nodes and subtraction vectors for the current pose."""
         nb = self.nb
-        self.r = self.pos - self.centres[self.bid]
-        self.e = []
+        r = self.pos - self.centres[self.bid]
+        if hasattr(self, "r"):
+            self.r.copy_(r)
+        else:
+            self.r = r
+        e_new = []
         for c in range(3):
             dens = torch.zeros_like(self.pos)
             dens[:, c] = self.w
@@ -205,7 +209,12 @@ class Bodies:
                 s = slice(k * nb, (k + 1) * nb)
                 pairsum("stresslet", self.pos[s], dens[s], self.pos[s], src_nrm=self.nrm[s],
                         eta=self.eta, reg=self.reg, eps=self.eps, out=e[s])
-            self.e.append(e)
+            e_new.append(e)
+        if hasattr(self, "e"):
+            for c in range(3):
+                self.e[c].copy_(e_new[c])
+        else:
+            self.e = e_new
 
     def pc_blocks(self):
         """The dense blocks [M - I_sub, -K; -K^T, I] of the current pose."""
@@ -312,14 +321,15 @@ class Bodies:
         kx = kx - kx.transpose(1, 2)
         Rinc = c[:, :, None] * torch.eye(3, dtype=torch.float64, device=W.device) \
             + s[:, :, None] * kx + (1 - c)[:, :, None] * ax[:, :, None] * ax[:, None, :]
-        self.rot = torch.bmm(Rinc, self.rot)
+        self.rot.copy_(torch.bmm(Rinc, self.rot))
         def rot(v):
             k = ax[self.bid]
             cc, ss = c[self.bid], s[self.bid]
             return v * cc + torch.cross(k, v, dim=1) * ss + k * (k * v).sum(1, keepdim=True) * (1 - cc)
-        self.centres = self.centres + U * dt
-        self.pos = self.centres[self.bid] + rot(self.r)
-        self.nrm = rot(self.nrm)
+        # in place: recorded graphs keep the addresses of these tensors
+        self.centres.add_(U * dt)
+        self.pos.copy_(self.centres[self.bid] + rot(self.r))
+        self.nrm.copy_(rot(self.nrm))
         self.update_geometry()
 
 
@@ -423,11 +433,17 @@ class FiberSystem:
         projection, explicit flows, LU factors (tstep.py:689-941)."""
         dev = self.dev
         E, dt, c0, c1, p = self.E, self.dt, self.c0, self.c1, self.penalty
-        A_buf = torch.empty(self.bs.nelem, dtype=torch.float64, device=dev)
-        F_buf = torch.empty(self.gF.nelem, dtype=torch.float64, device=dev)
-        Q_buf = [torch.empty(g.nelem, dtype=torch.float64, device=dev) for g in self.gQ]
-        self.weights = torch.empty(self.npts, dtype=torch.float64, device=dev)
-        self.xs = torch.empty((self.npts, 3), dtype=torch.float64, device=dev)
+        # Every buffer the matvec and the preconditioner read is allocated once
+        # and re-filled in place, so a recorded HIP graph of the Arnoldi
+        # operator stays valid from one step to the next.
+        if not hasattr(self, "A_buf"):
+            self.A_buf = torch.empty(self.bs.nelem, dtype=torch.float64, device=dev)
+            self.F_buf = torch.empty(self.gF.nelem, dtype=torch.float64, device=dev)
+            self.Q_buf = [torch.empty(g.nelem, dtype=torch.float64, device=dev) for g in self.gQ]
+            self.weights = torch.empty(self.npts, dtype=torch.float64, device=dev)
+            self.xs = torch.empty((self.npts, 3), dtype=torch.float64, device=dev)
+            self.xss = torch.empty((self.npts, 3), dtype=torch.float64, device=dev)
+        A_buf, F_buf, Q_buf = self.A_buf, self.F_buf, self.Q_buf
         self._rhs_parts = {}
         self._ctx = {}
         if self.assembly == "hip":
@@ -499,15 +515,16 @@ class FiberSystem:
             self.weights[node.reshape(-1)] = w.reshape(-1)
             self.xs[node.reshape(-1)] = xs.reshape(-1, 3)
             self._ctx[N] = (node, X, xs, xss, D1, P, B)
-        self.A_buf, self.F_buf, self.Q_buf = A_buf, F_buf, Q_buf
         self._build_rhs()
-        self.lu = BatchedLU(self.bs, self.A_buf, precision=self.lu_precision,
-                            refine=self.lu_refine, tol=self.lu_tol)
+        if getattr(self, "lu", None) is None:
+            self.lu = BatchedLU(self.bs, self.A_buf, precision=self.lu_precision,
+                                refine=self.lu_refine, tol=self.lu_tol)
+        else:
+            self.lu.factor()
 
     def _assemble_hip(self, A_buf, F_buf, Q_buf):
         """A_sq, force rows, tension projections, tangents and weights of every
         fiber from f3d_fiber_assemble, one launch per node count."""
-        self.xss = torch.empty((self.npts, 3), dtype=torch.float64, device=self.dev)
         lib = L.load()
         for N, bk in self.buckets.items():
             t = bk["asm"]

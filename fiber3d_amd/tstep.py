@@ -106,13 +106,13 @@ class TimeStepper:
         self.prev_solution = sol
         self.shell_density = sol[:3 * S.nper].clone()
         self.tension = sol[self.idx_t]
-        S.X = sol[self.idx_x].contiguous()
+        S.X.copy_(sol[self.idx_x])
         if S.bodies is not None:
             # rigid motion with the solved velocities (tstep.py:1398-1406)
             S.bodies.move(S.bodies._vel(sol, 3 * S.nper), S.dt)
+        # re-assembly refills the system's buffers in place: the recorded
+        # Arnoldi graphs stay valid
         S.prepare()
-        if self.graphs is not None:
-            self.graphs.reset()
         torch.cuda.synchronize()
         st = StepStats(res.iterations, res.restarts, res.residual, time.perf_counter() - t0,
                        res.guess_used)

@@ -82,3 +82,19 @@ def test_graph_time_steps_match_eager_steps():
         if graph:
             assert ts.graphs.captures >= 3
     assert rel2(runs[True], runs[False]) < 1e-12
+
+
+def test_graph_scratch_outlives_recording():
+    """The recorded operator's scratch buffer stays allocated: tensors created
+    after recording never share its memory (a replay writes it every call)."""
+    print("case graph scratch lifetime", flush=True)
+    fibers, bodies, _, shell = _small()
+    S = FiberSystem(fibers, shell, bodies=bodies, lu_precision="f32c", lu_refine=0)
+    op = ArnoldiGraphs(S).get("f32c")
+    scratch = op.keep[0].data_ptr()
+    fresh = [torch.zeros(S.size, dtype=torch.float64, device="cuda") for _ in range(64)]
+    assert all(t.data_ptr() != scratch for t in fresh)
+    for _ in range(3):
+        op(torch.from_numpy(np.random.default_rng(1).normal(size=S.size)).cuda())
+    torch.cuda.synchronize()
+    assert all(float(t.abs().max()) == 0.0 for t in fresh)
