@@ -67,8 +67,8 @@ def test_graph_gmres_reaches_fp64_tolerance():
 
 
 def test_graph_time_steps_match_eager_steps():
-    """Fast-mode steps with recorded operators track the eager steps; every
-    step's residual is the FP64 operator's."""
+    """Fast-mode steps with recorded operators track the eager steps over
+    several re-assemblies; every step's residual is the FP64 operator's."""
     print("case graph time steps", flush=True)
     fibers, _, _, shell = _small(with_bodies=False)
     runs = {}
@@ -80,7 +80,10 @@ def test_graph_time_steps_match_eager_steps():
         assert all(st.residual <= 1e-10 for st in stats)
         runs[graph] = ts.sys.X.cpu().numpy()
         if graph:
-            assert ts.graphs.captures >= 3
+            # recorded once, reused by every later step (assembly refills the
+            # recorded buffers in place)
+            assert 1 <= ts.graphs.captures <= 2
+            assert sum(op.replays for op in ts.graphs._ops.values()) >= 3 * 2
     assert rel2(runs[True], runs[False]) < 1e-12
 
 
