@@ -2,9 +2,12 @@
 
 Mirrors tstep.time_step_hydro (python/tstep/tstep.py:628-1216) for fibers with
 force/torque-free ends inside a spherical periphery:
-  * per-step assembly, batched over fibers of equal node count on the GPU
-    (fiber.form_linear_operator / compute_RHS / apply_BC_rectangular /
-    force_operator, python/fiber/fiber.py:322-961, :1399-1431);
+  * per-step assembly on the GPU (fiber.form_linear_operator / compute_RHS /
+    apply_BC_rectangular / force_operator, python/fiber/fiber.py:322-961,
+    :1399-1431): the fused f3d_fiber_assemble kernel, one launch per node
+    count (assembly='hip', default), or the same formulas as batched torch
+    operations (assembly='torch', kept as a second implementation to test
+    against);
   * the matvec A_body_fiber_hyd (tstep.py:1003-1178) from libf3d kernels:
     batched block GEMVs for the force operator, the tension projection and
     A_sq XT, the Oseen / stresslet pair sums with same-fiber exclusion
@@ -338,7 +341,7 @@ class FiberSystem:
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
                  lengths=None, pair_precision="f64", lu_precision="f64", lu_refine=6,
                  lu_tol=1e-13, reg=5e-3, eps=1e-5, bodies=None, body_force=None,
-                 assembly="torch"):
+                 assembly="hip"):
         self.dev = device()
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = _c0c1(epsilon, eta)
