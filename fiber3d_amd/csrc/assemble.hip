@@ -29,6 +29,64 @@ namespace f3d {
 constexpr int kAsmNmax = 64;
 constexpr int kAsmThreads = 256;
 
+// Coefficient of column jj, node k, of block (a, b) of A (before the
+// projection): the formulas of the file header.
+template <int A_BLK>
+__device__ __forceinline__ double a_coef(int b, int k, int jj, int N, const double* sxs,
+                                         const double* sxss, const double* sxsss,
+                                         const double* D01, const double* D02,
+                                         const double* D03, const double* D04, double s1,
+                                         double s2, double s3, double s4, double E, double dt,
+                                         double c0, double c1, double pen) {
+  if (A_BLK < 3) {
+    if (b < 3) {
+      const double xa = sxs[k * 3 + A_BLK], xb = sxs[k * 3 + b];
+      const double g = A_BLK == b ? c0 * (1.0 + xa * xa) + c1 * (1.0 - xa * xa)
+                                  : (c0 - c1) * xa * xb;
+      return E * g * s4 * D04[k * N + jj] + ((A_BLK == b && k == jj) ? 1.0 / dt : 0.0);
+    }
+    return -2.0 * c0 * sxs[k * 3 + A_BLK] * s1 * D01[k * N + jj] -
+           ((k == jj) ? (c0 + c1) * sxss[k * 3 + A_BLK] : 0.0);
+  }
+  if (b < 3)
+    return -(c1 + 7.0 * c0) * E * sxss[k * 3 + b] * s4 * D04[k * N + jj] -
+           6.0 * c0 * E * sxsss[k * 3 + b] * s3 * D03[k * N + jj] -
+           pen * sxs[k * 3 + b] * s1 * D01[k * N + jj];
+  const double x0 = sxss[k * 3], x1 = sxss[k * 3 + 1], x2 = sxss[k * 3 + 2];
+  return -2.0 * c0 * s2 * D02[k * N + jj] +
+         ((k == jj) ? (c1 + c0) * (x0 * x0 + x1 * x1 + x2 * x2) : 0.0);
+}
+
+// acc[i] = sum_k Pt[k][i] m_k over 64 (zero-padded) projection rows; the
+// transposed projection is read two rows at a time from LDS.
+#define F3D_ACC_ROWS(Pt, MEXPR)                                                  \
+  _Pragma("unroll") for (int i = 0; i < kAsmNmax; ++i) acc[i] = 0.0;           \
+  for (int k = 0; k < N; ++k) {                                                \
+    const double m = (MEXPR);                                                  \
+    const double2* __restrict__ p2 = reinterpret_cast<const double2*>((Pt) + k * kAsmNmax); \
+    _Pragma("unroll") for (int i2 = 0; i2 < kAsmNmax / 2; ++i2) {              \
+      const double2 pp = p2[i2];                                               \
+      acc[2 * i2] = fma(pp.x, m, acc[2 * i2]);                                 \
+      acc[2 * i2 + 1] = fma(pp.y, m, acc[2 * i2 + 1]);                         \
+    }                                                                          \
+  }
+
+template <int A_BLK>
+__device__ __forceinline__ void a_block(double (&acc)[kAsmNmax], const double* sPt, int b, int jj,
+                                        int N, int rows, double* __restrict__ Acol, int r0,
+                                        const double* sxs, const double* sxss,
+                                        const double* sxsss, const double* D01,
+                                        const double* D02, const double* D03,
+                                        const double* D04, double s1, double s2, double s3,
+                                        double s4, double E, double dt, double c0, double c1,
+                                        double pen) {
+  F3D_ACC_ROWS(sPt, (a_coef<A_BLK>(b, k, jj, N, sxs, sxss, sxsss, D01, D02, D03, D04, s1, s2,
+                                   s3, s4, E, dt, c0, c1, pen)))
+#pragma unroll
+  for (int i = 0; i < kAsmNmax; ++i)
+    if (i < rows) Acol[r0 + i] = acc[i];
+}
+
 __global__ void __launch_bounds__(kAsmThreads)
 fiber_assemble_kernel(int nfib, int N, const long* __restrict__ node0,
                       const double* __restrict__ length, const long* __restrict__ a_off,
@@ -40,14 +98,18 @@ fiber_assemble_kernel(int nfib, int N, const long* __restrict__ node0,
                       double* __restrict__ Q0, double* __restrict__ Q1, double* __restrict__ Q2,
                       double* __restrict__ xs_out, double* __restrict__ xss_out,
                       double* __restrict__ w_out) {
-  __shared__ double sPX[(kAsmNmax - 4) * kAsmNmax];  // row i at i * N
-  __shared__ double sPT[(kAsmNmax - 2) * kAsmNmax];
+  // projections transposed ([k][i]) and zero-padded to 64 rows
+  __shared__ __align__(16) double sPXt[kAsmNmax * kAsmNmax];
+  __shared__ __align__(16) double sPTt[kAsmNmax * kAsmNmax];
   __shared__ double sX[kAsmNmax * 3], sxs[kAsmNmax * 3], sxss[kAsmNmax * 3], sxsss[kAsmNmax * 3];
 
   const int t = threadIdx.x;
   const int nx = N - 4, nt = N - 2, n4 = 4 * N;
-  for (int i = t; i < nx * N; i += kAsmThreads) sPX[i] = PX[i];
-  for (int i = t; i < nt * N; i += kAsmThreads) sPT[i] = PT[i];
+  for (int e = t; e < kAsmNmax * kAsmNmax; e += kAsmThreads) {
+    const int k = e / kAsmNmax, i = e % kAsmNmax;
+    sPXt[e] = (k < N && i < nx) ? PX[i * N + k] : 0.0;
+    sPTt[e] = (k < N && i < nt) ? PT[i * N + k] : 0.0;
+  }
   const double* __restrict__ D01 = D0;
   const double* __restrict__ D02 = D0 + (long)N * N;
   const double* __restrict__ D03 = D0 + 2l * N * N;
@@ -79,44 +141,17 @@ fiber_assemble_kernel(int nfib, int N, const long* __restrict__ node0,
     if (t < N) w_out[nd + t] = w0[t] * (0.5 * length[f]);
     __syncthreads();
 
+    double acc[kAsmNmax];
     // ---- A_sq and F, column j -------------------------------------------
     if (t < n4) {
       const int j = t, b = j / N, jj = j % N;
       double* __restrict__ Acol = Aout + a_off[f] + (long)j * n4;
-      for (int a = 0; a < 4; ++a) {
-        const int rows = a < 3 ? nx : nt;
-        const double* __restrict__ Pa = a < 3 ? sPX : sPT;
-        double acc[kAsmNmax - 2];
-#pragma unroll
-        for (int i = 0; i < kAsmNmax - 2; ++i) acc[i] = 0.0;
-        for (int k = 0; k < N; ++k) {
-          double m;
-          if (a < 3 && b < 3) {
-            const double xa = sxs[k * 3 + a], xb = sxs[k * 3 + b];
-            const double g = a == b ? c0 * (1.0 + xa * xa) + c1 * (1.0 - xa * xa)
-                                    : (c0 - c1) * xa * xb;
-            m = E * g * s4 * D04[k * N + jj] + ((a == b && k == jj) ? 1.0 / dt : 0.0);
-          } else if (a < 3) {  // b == T
-            m = -2.0 * c0 * sxs[k * 3 + a] * s1 * D01[k * N + jj] -
-                ((k == jj) ? (c0 + c1) * sxss[k * 3 + a] : 0.0);
-          } else if (b < 3) {  // a == T
-            m = -(c1 + 7.0 * c0) * E * sxss[k * 3 + b] * s4 * D04[k * N + jj] -
-                6.0 * c0 * E * sxsss[k * 3 + b] * s3 * D03[k * N + jj] -
-                pen * sxs[k * 3 + b] * s1 * D01[k * N + jj];
-          } else {
-            const double x0 = sxss[k * 3], x1 = sxss[k * 3 + 1], x2 = sxss[k * 3 + 2];
-            m = -2.0 * c0 * s2 * D02[k * N + jj] +
-                ((k == jj) ? (c1 + c0) * (x0 * x0 + x1 * x1 + x2 * x2) : 0.0);
-          }
-#pragma unroll
-          for (int i = 0; i < kAsmNmax - 2; ++i)
-            if (i < rows) acc[i] = fma(Pa[i * N + k], m, acc[i]);
-        }
-        const int r0 = a * nx;
-#pragma unroll
-        for (int i = 0; i < kAsmNmax - 2; ++i)
-          if (i < rows) Acol[r0 + i] = acc[i];
-      }
+#define F3D_ARGS sxs, sxss, sxsss, D01, D02, D03, D04, s1, s2, s3, s4, E, dt, c0, c1, pen
+      a_block<0>(acc, sPXt, b, jj, N, nx, Acol, 0, F3D_ARGS);
+      a_block<1>(acc, sPXt, b, jj, N, nx, Acol, nx, F3D_ARGS);
+      a_block<2>(acc, sPXt, b, jj, N, nx, Acol, 2 * nx, F3D_ARGS);
+      a_block<3>(acc, sPTt, b, jj, N, nt, Acol, 3 * nx, F3D_ARGS);
+#undef F3D_ARGS
       // 14 boundary rows (force- and torque-free ends)
       double br[14];
 #pragma unroll
@@ -151,11 +186,13 @@ fiber_assemble_kernel(int nfib, int N, const long* __restrict__ node0,
       double* __restrict__ Fb = Fout + f_off[f];
       for (int a = 0; a < 3; ++a) {
         double* __restrict__ Fcol = Fb + (long)a * N * n4 + (long)j * N;
-        for (int i = 0; i < N; ++i) {
-          double v = 0.0;
-          if (b == a) v = -E * s4 * D04[i * N + jj];
-          else if (b == 3) v = sxs[i * 3 + a] * s1 * D01[i * N + jj] + (i == jj ? sxss[i * 3 + a] : 0.0);
-          Fcol[i] = v;
+        if (b == a) {
+          for (int i = 0; i < N; ++i) Fcol[i] = -E * s4 * D04[i * N + jj];
+        } else if (b == 3) {
+          for (int i = 0; i < N; ++i)
+            Fcol[i] = sxs[i * 3 + a] * s1 * D01[i * N + jj] + (i == jj ? sxss[i * 3 + a] : 0.0);
+        } else {
+          for (int i = 0; i < N; ++i) Fcol[i] = 0.0;
         }
       }
     }
@@ -165,15 +202,15 @@ fiber_assemble_kernel(int nfib, int N, const long* __restrict__ node0,
       double* __restrict__ Q = c == 0 ? Q0 : (c == 1 ? Q1 : Q2);
       double* __restrict__ Qcol = Q + q_off[3 * f + c] + (long)jj * (n4 - 14);
       for (int r = 0; r < 3 * nx; ++r) Qcol[r] = 0.0;
-      for (int i = 0; i < nx; ++i) Qcol[c * nx + i] = sPX[i * N + jj];
-      for (int i = 0; i < nt; ++i) {
-        double acc = 0.0;
-        for (int k = 0; k < N; ++k) acc = fma(sPT[i * N + k], sxs[k * 3 + c] * s1 * D01[k * N + jj], acc);
-        Qcol[3 * nx + i] = acc;
-      }
+      for (int i = 0; i < nx; ++i) Qcol[c * nx + i] = sPXt[jj * kAsmNmax + i];
+      F3D_ACC_ROWS(sPTt, sxs[k * 3 + c] * s1 * D01[k * N + jj])
+#pragma unroll
+      for (int i = 0; i < kAsmNmax; ++i)
+        if (i < nt) Qcol[3 * nx + i] = acc[i];
     }
   }
 }
+#undef F3D_ACC_ROWS
 
 }  // namespace f3d
 
