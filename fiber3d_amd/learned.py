@@ -140,7 +140,10 @@ class LearnedGuess:
     """
 
     def __init__(self, train_steps=100, epochs=200, lr=1e-3, batch=8192, seed=0,
-                 use_net=True, gate=0.9, val_frac=0.2):
+                 use_net=True, gate=0.9, val_frac=0.2, order=1):
+        if order not in (1, 2):
+            raise ValueError("extrapolation order is 1 or 2")
+        self.order = order
         self.train_steps = train_steps
         self.epochs, self.lr, self.batch, self.seed = epochs, lr, batch, seed
         self.use_net = use_net
@@ -190,6 +193,10 @@ class LearnedGuess:
         return u, inv
 
     def _extrapolated(self):
+        """Linear (2 p1 - p2) or, with order=2 and three solutions on hand,
+        quadratic (3 p1 - 3 p2 + p3) extrapolation in time."""
+        if self.order == 2 and len(self.prev) >= 3:
+            return 3.0 * (self.prev[-1] - self.prev[-2]) + self.prev[-3]
         return 2.0 * self.prev[-1] - self.prev[-2]
 
     # ---- predictor protocol ------------------------------------------------

@@ -24,8 +24,6 @@ force/torque-free ends inside a spherical periphery:
 Unknown vector layout is the reference's: [shell density (3 Nper, AoS) |
 per body density (3 Nb, AoS), U, Omega | per fiber X(N), Y(N), Z(N), T(N)].
 """
-import ctypes
-
 import numpy as np
 import torch
 

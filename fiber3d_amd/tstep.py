@@ -23,7 +23,7 @@ import torch
 
 from .gmres import gmres
 from .graphs import ArnoldiGraphs
-from .system import FiberSystem, Shell
+from .system import FiberSystem
 
 
 @dataclass

@@ -5,7 +5,6 @@ identity or an independent form, chosen so that a wrong constant, a dropped
 term, a wrong sign or a transposed operand fails.
 """
 import numpy as np
-import pytest
 
 from conftest import rel2
 from fiber3d_amd.geometry import shape_gallery_sphere
