@@ -24,6 +24,18 @@ reached its accuracy floor and the remaining cycles use `matvec`.
 `fused_inner` / `fused_matvec`, when given, compute v -> A P^-1 v in one call
 (e.g. a recorded HIP graph of the preconditioner and the operator) and replace
 the separate precond + operator calls of the Arnoldi step.
+
+`update_residual` (with an inner operator) replaces the FP64 product that
+ends an inner cycle by an update of the cycle's FP64 starting residual r:
+the correction's image under the inner operator is V_{k+1} H y by the Arnoldi
+relation, so r_new = r - V_{k+1} (H y) costs one small GEMV with the basis.
+It differs from the true b - A x_new only by the inner operator's error on
+the correction, at most `inner_rel_err` * ||H y||.  The cycle is accepted as
+converged when ||r_new|| + inner_rel_err * ||H y|| <= tol ||b||; otherwise
+the FP64 residual is computed as usual.  From a zero guess ||H y|| ~ ||b||,
+so the bound never passes and the FP64 product is kept; from a close guess
+||H y|| ~ ||b - A x0||, and one FP64 product per solve (the guess's residual)
+remains.
 """
 import math
 from dataclasses import dataclass
@@ -42,10 +54,13 @@ class GmresResult:
     guess_used: bool
     history: list
     inner_cycles: int = 0    # restart cycles run with the inner operator
+    residual_kind: str = "fp64"  # "fp64": b - A x recomputed; "update": FP64-anchored update
+    residual_bound: float = 0.0  # bound on the true relative residual
 
 
 def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, inner=None,
-          callback=None, fused_inner=None, fused_matvec=None):
+          callback=None, fused_inner=None, fused_matvec=None, update_residual=False,
+          inner_rel_err=1e-6):
     Pinv = precond if precond is not None else (lambda v, out=None: v.clone())
     op = inner if inner is not None else matvec
     inner_cycles = 0
@@ -76,9 +91,10 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
         hist.append(beta / bnorm)
         if beta <= tol * bnorm:
             return GmresResult(x, 0, total, restarts, beta / bnorm, guess_used, hist,
-                               inner_cycles)
+                               inner_cycles, "fp64", beta / bnorm)
         V[0] = r / beta
         H = np.zeros((m + 1, m))
+        Hraw = np.zeros((m + 1, m))  # unrotated Hessenberg: A P^-1 V_k = V_{k+1} Hraw
         cs = np.zeros(m)
         sn = np.zeros(m)
         g = np.zeros(m + 1)
@@ -100,6 +116,7 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
             hn = torch.linalg.norm(w)
             hcol = torch.cat((h1 + h2, hn.reshape(1))).cpu().numpy()
             H[:j + 2, j] = hcol
+            Hraw[:j + 2, j] = hcol
             if hcol[j + 1] > 0:
                 V[j + 1] = w / hcol[j + 1]
             for i in range(j):
@@ -123,12 +140,22 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
         yk = np.linalg.solve(np.triu(H[:j_done, :j_done]), g[:j_done]) if j_done else np.zeros(0)
         upd = torch.mv(V[:j_done].t(), torch.from_numpy(yk).to(dev))
         x += Pinv(upd)
-        r = b - matvec(x)
         restarts += 1
+        if update_residual and op is not matvec and j_done and est <= tol:
+            hy = Hraw[:j_done + 1, :j_done] @ yk
+            k1 = j_done + 1 if hy[j_done] != 0.0 else j_done  # V[j_done] unset at breakdown
+            r_upd = r - torch.mv(V[:k1].t(), torch.from_numpy(hy[:k1]).to(dev))
+            rn_upd = float(torch.linalg.norm(r_upd))
+            bound = rn_upd + inner_rel_err * float(np.linalg.norm(hy))
+            if bound <= tol * bnorm:
+                hist.append(rn_upd / bnorm)
+                return GmresResult(x, 0, total, restarts, rn_upd / bnorm, guess_used, hist,
+                                   inner_cycles + 1, "update", bound / bnorm)
+        r = b - matvec(x)
         if op is not matvec:
             inner_cycles += 1
             if est <= tol and float(torch.linalg.norm(r)) > tol * bnorm:
                 op = matvec
     rn = float(torch.linalg.norm(r)) / bnorm
     return GmresResult(x, 0 if rn <= tol else total, total, restarts, rn, guess_used, hist,
-                       inner_cycles)
+                       inner_cycles, "fp64", rn)

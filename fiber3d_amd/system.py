@@ -310,6 +310,16 @@ class Bodies:
         self.block(out, start).view(self.count, k, 3).copy_(
             y[..., 0:1] * R[..., :, 0] + y[..., 1:2] * R[..., :, 1] + y[..., 2:3] * R[..., :, 2])
 
+    def state(self):
+        """Copies of the pose (centres, nodes, normals, rotation)."""
+        return tuple(t.clone() for t in (self.centres, self.pos, self.nrm, self.rot))
+
+    def set_state(self, st):
+        """Restore a pose taken by state(), in place."""
+        for t, v in zip((self.centres, self.pos, self.nrm, self.rot), st):
+            t.copy_(v)
+        self.update_geometry()
+
     def move(self, vel, dt):
         """Rigid update: centre += U dt, nodes rotated by Omega dt (Rodrigues)."""
         U, W = vel[:, :3], vel[:, 3:]
@@ -429,9 +439,13 @@ class FiberSystem:
         """(npts, 3) view of an SoA (3, npts) buffer."""
         return buf.view(3, self.npts).t()
 
-    def prepare(self):
+    def prepare(self, refactor=True):
         """Per-step assembly: derivatives, A_sq, RHS, force operator, tension
-        projection, explicit flows, LU factors (tstep.py:689-941)."""
+        projection, explicit flows, LU factors (tstep.py:689-941).
+
+        refactor=False keeps the fiber LU factors of an earlier assembly as the
+        preconditioner (a lagged preconditioner: GMRES still solves with the
+        current operator, and the current A_sq is what any refinement uses)."""
         dev = self.dev
         E, dt, c0, c1, p = self.E, self.dt, self.c0, self.c1, self.penalty
         # Every buffer the matvec and the preconditioner read is allocated once
@@ -520,7 +534,7 @@ class FiberSystem:
         if getattr(self, "lu", None) is None:
             self.lu = BatchedLU(self.bs, self.A_buf, precision=self.lu_precision,
                                 refine=self.lu_refine, tol=self.lu_tol)
-        else:
+        elif refactor:
             self.lu.factor()
 
     def _assemble_hip(self, A_buf, F_buf, Q_buf):

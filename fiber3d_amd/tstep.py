@@ -14,6 +14,14 @@ Two modes:
   'fast'   : FP32-compensated pair sums inside the Krylov iteration with the
              FP64 operator deciding convergence, FP32 fiber LU with FP64
              refinement, and an optional initial-guess predictor.
+
+In fast mode the closing residual is, by default (certify="update"), the
+guess's FP64 residual updated by the correction's image under the
+FP32-compensated operator, accepted only under a bound that includes that
+operator's error (fiber3d_amd.gmres); certify="fp64" recomputes b - A x in
+FP64 instead.  With record=True every step keeps its geometry, right-hand
+side and solution, and verify() re-assembles each recorded step and returns
+the true FP64 residuals.
 """
 import time
 from dataclasses import dataclass
@@ -33,16 +41,27 @@ class StepStats:
     residual: float
     seconds: float
     guess_used: bool
+    residual_kind: str = "fp64"
+    residual_bound: float = 0.0
 
 
 class TimeStepper:
     def __init__(self, fibers, shell, mode="plain", eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference", tol=1e-8,
                  restart=150, maxiter=200, predictor=None, lengths=None, bodies=None,
-                 body_force=None, graph=False):
+                 body_force=None, graph=False, certify="update", record=False, lu_every=1):
         if mode not in ("plain", "fast"):
             raise ValueError("mode is 'plain' or 'fast'")
+        if certify not in ("update", "fp64"):
+            raise ValueError("certify is 'update' or 'fp64'")
         self.mode = mode
+        self.certify = certify
+        self.record = record
+        self.records = []
+        # fiber LU factors are recomputed every lu_every-th assembly and
+        # reused as a lagged preconditioner in between
+        self.lu_every = max(1, int(lu_every))
+        self.nsteps = 0
         self.tol, self.restart, self.maxiter = tol, restart, maxiter
         fast = mode == "fast"
         self.sys = FiberSystem(fibers, shell, eta=eta, dt=dt, E=E, epsilon=epsilon,
@@ -99,8 +118,13 @@ class TimeStepper:
             if self.mode == "fast":
                 fused["fused_inner"] = lambda v, out: G.get("f32c")(v, out=out)
         res = gmres(S.matvec, S.rhs, S.precond, tol=self.tol, restart=self.restart,
-                    maxiter=self.maxiter, x0=x0, inner=self.inner_operator(), **fused)
+                    maxiter=self.maxiter, x0=x0, inner=self.inner_operator(),
+                    update_residual=self.mode == "fast" and self.certify == "update", **fused)
         sol = res.x
+        if self.record:
+            # S.rhs and sol are fresh tensors of this step; X is updated in place
+            self.records.append((S.X.clone(), S.bodies.state() if S.bodies is not None else None,
+                                 S.rhs, sol))
         if self.predictor is not None:
             self.predictor.observe(self, sol)
         self.prev_solution = sol
@@ -112,9 +136,32 @@ class TimeStepper:
             S.bodies.move(S.bodies._vel(sol, 3 * S.nper), S.dt)
         # re-assembly refills the system's buffers in place: the recorded
         # Arnoldi graphs stay valid
-        S.prepare()
+        self.nsteps += 1
+        S.prepare(refactor=self.nsteps % self.lu_every == 0)
         torch.cuda.synchronize()
         st = StepStats(res.iterations, res.restarts, res.residual, time.perf_counter() - t0,
-                       res.guess_used)
+                       res.guess_used, res.residual_kind, res.residual_bound)
         self.history.append(st)
         return st
+
+    def verify(self):
+        """True relative residuals ||b - A x|| / ||b|| of the recorded steps in
+        FP64, each with the system re-assembled at that step's geometry, and
+        the largest difference between the re-assembled and the recorded
+        right-hand sides.  The current state is restored afterwards."""
+        S = self.sys
+        now = (S.X.clone(), S.bodies.state() if S.bodies is not None else None)
+        out, drhs = [], 0.0
+        for X, body, b, x in self.records:
+            S.X.copy_(X)
+            if body is not None:
+                S.bodies.set_state(body)
+            S.prepare()
+            drhs = max(drhs, float((S.rhs - b).abs().max()))
+            r = b - S.matvec(x, precision="f64")
+            out.append(float(torch.linalg.norm(r) / torch.linalg.norm(b)))
+        S.X.copy_(now[0])
+        if now[1] is not None:
+            S.bodies.set_state(now[1])
+        S.prepare()
+        return out, drhs

@@ -97,3 +97,56 @@ def test_zero_rhs():
     A, b, mv, pc, bt = _system(seed=5)
     res = gmres(mv, torch.zeros_like(bt), pc, tol=1e-10)
     assert res.info == 0 and res.iterations == 0 and float(res.x.abs().max()) == 0.0
+
+
+def _inexact(A, rel, seed):
+    rng = np.random.default_rng(seed)
+    Ai = torch.from_numpy(A + rng.normal(size=A.shape) * rel * np.abs(A).max())
+
+    def inner(v, out=None):
+        y = torch.mv(Ai, v)
+        if out is None:
+            return y
+        out.copy_(y)
+        return out
+    return inner
+
+
+def test_residual_update_from_a_close_guess_needs_one_fp64_product():
+    """From a guess with a small residual, the FP64-anchored update replaces
+    the closing FP64 product; its bound holds for the true residual."""
+    A, b, mv, pc, bt = _system(seed=6)
+    x = np.linalg.solve(A, b)
+    calls = {"n": 0}
+
+    def counted(v, out=None):
+        calls["n"] += 1
+        return mv(v, out=out)
+    x0 = torch.from_numpy(x + 1e-6 * np.random.default_rng(1).normal(size=x.shape))
+    res = gmres(counted, bt, pc, tol=1e-10, x0=x0, inner=_inexact(A, 1e-7, 10),
+                update_residual=True)
+    assert res.info == 0 and res.residual_kind == "update" and calls["n"] == 1
+    true = np.linalg.norm(b - A @ res.x.numpy()) / np.linalg.norm(b)
+    assert true <= res.residual_bound <= 1e-10
+    assert abs(true - res.residual) <= 1e-6 * 1e-4     # inner error times ||b - A x0|| / ||b||
+
+
+def test_residual_update_falls_back_to_fp64_from_zero():
+    """From a zero guess the correction is as large as b, the inner operator's
+    error on it exceeds the tolerance, and the FP64 residual decides."""
+    A, b, mv, pc, bt = _system(seed=7)
+    res = gmres(mv, bt, pc, tol=1e-10, inner=_inexact(A, 1e-7, 11), update_residual=True)
+    assert res.info == 0 and res.residual_kind == "fp64" and res.residual <= 1e-10
+    assert np.linalg.norm(b - A @ res.x.numpy()) / np.linalg.norm(b) <= 1.01e-10
+
+
+def test_residual_update_rejected_when_bound_fails():
+    """A large declared inner error makes the bound fail even from a close
+    guess: the FP64 product is computed and the result stays correct."""
+    A, b, mv, pc, bt = _system(seed=8)
+    x = np.linalg.solve(A, b)
+    x0 = torch.from_numpy(x + 1e-6 * np.random.default_rng(2).normal(size=x.shape))
+    res = gmres(mv, bt, pc, tol=1e-10, x0=x0, inner=_inexact(A, 1e-7, 12),
+                update_residual=True, inner_rel_err=1.0)
+    assert res.info == 0 and res.residual_kind == "fp64"
+    assert np.linalg.norm(b - A @ res.x.numpy()) / np.linalg.norm(b) <= 1.01e-10
