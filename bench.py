@@ -128,7 +128,7 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
         its = [st.iterations for st in stats]
         out[name] = {"time_steps_per_s": round(steps / dt, 4), "mean_gmres_iters": float(np.mean(its)),
                      "max_true_residual": float(max(st.residual for st in stats)),
-                     "steps": steps, "untimed_steps": warm}
+                     "steps": steps, "untimed_steps": warm, "lu_refactor_every": ts.lu_every}
         if ts.record:
             true, drhs = ts.verify()
             kinds = [st.residual_kind for st in stats]

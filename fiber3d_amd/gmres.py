@@ -60,7 +60,10 @@ class GmresResult:
 
 def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, inner=None,
           callback=None, fused_inner=None, fused_matvec=None, update_residual=False,
-          inner_rel_err=1e-6):
+          inner_rel_err=1e-6, target=None):
+    # iterations continue to `target` (<= tol) once started; a guess (or any
+    # restart) whose FP64 residual is already <= tol is accepted as it is
+    t_it = tol if target is None else min(float(target), tol)
     Pinv = precond if precond is not None else (lambda v, out=None: v.clone())
     op = inner if inner is not None else matvec
     inner_cycles = 0
@@ -135,13 +138,13 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
             hist.append(est)
             if callback is not None:
                 callback(est)
-            if est <= tol or hcol[j + 1] == 0.0:
+            if est <= t_it or hcol[j + 1] == 0.0:
                 break
         yk = np.linalg.solve(np.triu(H[:j_done, :j_done]), g[:j_done]) if j_done else np.zeros(0)
         upd = torch.mv(V[:j_done].t(), torch.from_numpy(yk).to(dev))
         x += Pinv(upd)
         restarts += 1
-        if update_residual and op is not matvec and j_done and est <= tol:
+        if update_residual and op is not matvec and j_done and est <= t_it:
             hy = Hraw[:j_done + 1, :j_done] @ yk
             k1 = j_done + 1 if hy[j_done] != 0.0 else j_done  # V[j_done] unset at breakdown
             r_upd = r - torch.mv(V[:k1].t(), torch.from_numpy(hy[:k1]).to(dev))
@@ -154,7 +157,7 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
         r = b - matvec(x)
         if op is not matvec:
             inner_cycles += 1
-            if est <= tol and float(torch.linalg.norm(r)) > tol * bnorm:
+            if est <= t_it and float(torch.linalg.norm(r)) > tol * bnorm:
                 op = matvec
     rn = float(torch.linalg.norm(r)) / bnorm
     return GmresResult(x, 0 if rn <= tol else total, total, restarts, rn, guess_used, hist,

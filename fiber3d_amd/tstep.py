@@ -49,7 +49,8 @@ class TimeStepper:
     def __init__(self, fibers, shell, mode="plain", eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference", tol=1e-8,
                  restart=150, maxiter=200, predictor=None, lengths=None, bodies=None,
-                 body_force=None, graph=False, certify="update", record=False, lu_every=1):
+                 body_force=None, graph=False, certify="update", record=False, lu_every=None,
+                 target=None):
         if mode not in ("plain", "fast"):
             raise ValueError("mode is 'plain' or 'fast'")
         if certify not in ("update", "fp64"):
@@ -59,8 +60,10 @@ class TimeStepper:
         self.record = record
         self.records = []
         # fiber LU factors are recomputed every lu_every-th assembly and
-        # reused as a lagged preconditioner in between
-        self.lu_every = max(1, int(lu_every))
+        # reused as a lagged preconditioner in between (fast mode: every 8th)
+        self.lu_every = max(1, int(lu_every if lu_every is not None
+                                   else (8 if mode == "fast" else 1)))
+        self.target = target
         self.nsteps = 0
         self.tol, self.restart, self.maxiter = tol, restart, maxiter
         fast = mode == "fast"
@@ -119,7 +122,8 @@ class TimeStepper:
                 fused["fused_inner"] = lambda v, out: G.get("f32c")(v, out=out)
         res = gmres(S.matvec, S.rhs, S.precond, tol=self.tol, restart=self.restart,
                     maxiter=self.maxiter, x0=x0, inner=self.inner_operator(),
-                    update_residual=self.mode == "fast" and self.certify == "update", **fused)
+                    update_residual=self.mode == "fast" and self.certify == "update",
+                    target=self.target, **fused)
         sol = res.x
         if self.record:
             # S.rhs and sol are fresh tensors of this step; X is updated in place

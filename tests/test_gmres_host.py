@@ -150,3 +150,17 @@ def test_residual_update_rejected_when_bound_fails():
                 update_residual=True, inner_rel_err=1.0)
     assert res.info == 0 and res.residual_kind == "fp64"
     assert np.linalg.norm(b - A @ res.x.numpy()) / np.linalg.norm(b) <= 1.01e-10
+
+
+def test_target_below_tolerance():
+    """Once iterating, GMRES continues to `target`; a guess whose residual
+    is already within tol is accepted with no iteration."""
+    A, b, mv, pc, bt = _system(seed=9)
+    x = np.linalg.solve(A, b)
+    plain = gmres(mv, bt, pc, tol=1e-8)
+    deep = gmres(mv, bt, pc, tol=1e-8, target=1e-11)
+    assert deep.residual <= 1e-11 < plain.residual <= 1e-8
+    assert deep.iterations > plain.iterations
+    near = torch.from_numpy(x * (1 + 1e-12))
+    res = gmres(mv, bt, pc, tol=1e-8, target=1e-11, x0=near)
+    assert res.iterations == 0 and res.guess_used and res.residual <= 1e-8
