@@ -149,6 +149,62 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
     return out
 
 
+def small_leg(steps=30, cpu=True):
+    """The small workload (32 fibers x 32 nodes, 1024-node periphery): GPU
+    time-steps/s, plain FP64 and fast with the extrapolated guess, beside one
+    step of the oracle's FP64 restatement on the host (NumPy/SciPy: assembly,
+    per-fiber and shell LU, GMRES; the one-time shell matrix and its LU are
+    part of the oracle's set-up and are included)."""
+    import torch
+    from fiber3d_amd import configs
+    from fiber3d_amd.learned import LearnedGuess
+    from fiber3d_amd.system import Shell
+    from fiber3d_amd.tstep import TimeStepper
+
+    cfg = configs.config(0)
+    spec = cfg.fibers
+    fibers = configs.make_fibers(spec, cfg.seed)
+    xs, ns, ws = configs.make_periphery(cfg.periphery_nodes, cfg.periphery_radius)
+    shell = Shell(xs, ns, ws, eta=cfg.eta)
+    out = {"workload": f"{cfg.name}: {len(fibers)} fibers x {spec.num_points} nodes, "
+                       f"{cfg.periphery_nodes}-node periphery, dt={cfg.dt}, GMRES tol {cfg.gmres_tol}"}
+    for name, mode, pred in (("plain", "plain", None),
+                             ("extrapolated_fast", "fast",
+                              LearnedGuess(train_steps=10**9, use_net=False))):
+        ts = TimeStepper(fibers, shell, mode=mode, eta=cfg.eta, dt=cfg.dt, E=spec.E,
+                         epsilon=spec.epsilon, tol=cfg.gmres_tol, restart=cfg.restart,
+                         maxiter=cfg.maxiter, predictor=pred,
+                         lengths=np.full(len(fibers), spec.length))
+        for _ in range(4):
+            ts.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = [ts.step() for _ in range(steps)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"time_steps_per_s": round(steps / dt, 2),
+                     "mean_gmres_iters": float(np.mean([st.iterations for st in stats])),
+                     "max_residual": float(max(st.residual for st in stats))}
+    if cpu:
+        from oracle import tstep as OT
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        ora = OT.HydroSystem(fibers, (xs, ns, ws), eta=cfg.eta, dt=cfg.dt, E=spec.E,
+                             epsilon=spec.epsilon, lengths=np.full(len(fibers), spec.length))
+        t1 = time.perf_counter()
+        xo, info, its = OT.gmres(ora.matvec, ora.rhs, ora.precond, tol=cfg.gmres_tol,
+                                 restart=cfg.restart, maxiter=cfg.maxiter)
+        t2 = time.perf_counter()
+        res = float(np.linalg.norm(ora.rhs - ora.matvec(xo)) / np.linalg.norm(ora.rhs))
+        out["cpu_baseline"] = {"value": round(1.0 / (t2 - t0), 4), "unit": "time-steps/s",
+                               "cores": threads, "kind": "port",
+                               "sample": "one plain FP64 step of this workload by the oracle "
+                                         "(NumPy/SciPy restatement of the reference's step)",
+                               "setup_s": round(t1 - t0, 3), "solve_s": round(t2 - t1, 3),
+                               "gmres_iters": int(its), "residual": res}
+    return out
+
+
 def lu_leg(reps=5, cpu=True):
     """Batched fiber preconditioner at workload 2: 2048 fibers, N in
     {16,24,32,48,64} (block order n = 4N), A_sq from the fiber discretisation.
@@ -282,11 +338,13 @@ def main():
     ap.add_argument("--system", type=int, default=4,
                     help="also time implicit steps of fiber-system workload 0, 3 or 4 "
                          "(-1: raw sweep only)")
-    ap.add_argument("--system-steps", type=int, default=4)
+    ap.add_argument("--system-steps", type=int, default=20)
     ap.add_argument("--train-steps", type=int, default=8,
                     help="untimed steps the learned predictor trains on (0: plain and fast only)")
     ap.add_argument("--no-lu", dest="lu", action="store_false",
                     help="skip the batched fiber LU (workload 2) and MLP GEMM legs")
+    ap.add_argument("--no-small", dest="small", action="store_false",
+                    help="skip the small-workload time steps (GPU and oracle on the host)")
     ap.add_argument("--shell-pc", default="auto", choices=["auto", "dense", "block"])
     ap.add_argument("--graph", action="store_true",
                     help="replay the Arnoldi operator as a recorded HIP graph")
@@ -382,6 +440,8 @@ def main():
     if args.lu:
         line["batched_lu"] = lu_leg(cpu=not args.no_cpu)
         line["mlp_gemm_bf16"] = mlp_leg()
+    if args.small:
+        line["fiber_system_small"] = small_leg(cpu=not args.no_cpu)
     if args.system >= 0:
         line["fiber_system"] = system_leg(args.system, args.system_steps, args.train_steps,
                                           shell_pc=args.shell_pc, graph=args.graph)

@@ -63,6 +63,7 @@ SIGNATURES = {
     "f3d_fiber_assemble": (ctypes.c_int, [
         _i64, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
         _f64, _f64, _f64, _f64, _f64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "f3d_dense_gemv_batched": (ctypes.c_int, [_i64, _i32, _c_p, _c_p, _c_p, _c_p]),
     "f3d_block_gemv_batched": (ctypes.c_int, [
         _i64, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _f64, _f64, _c_p]),
 }

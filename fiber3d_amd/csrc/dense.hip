@@ -748,6 +748,53 @@ block_gemv_kernel(const int* __restrict__ m_arr, const int* __restrict__ k_arr,
   }
 }
 
+// ---------------------------------------------------------------------------
+// y_b = M_b x_b for `count` dense ROW-major m x m FP64 matrices, one wave per
+// row (four rows per 256-thread workgroup).  A wave reads its row in 1 KB
+// coalesced steps (16 bytes per lane when `pairs` is set: m even, M and x
+// 16-byte aligned), accumulates in two FP64 chains and reduces across the
+// wave in a fixed order, so the result is deterministic.  Even one matrix of
+// a few thousand rows puts thousands of waves in flight, which an HBM-bound
+// sweep over a large inverse needs; the column-major row-block GEMV above
+// runs one wave per 256-row block.
+__global__ void __launch_bounds__(256)
+dense_gemv_kernel(int64_t rows_total, int32_t m, int pairs, const double* __restrict__ M,
+                  const double* __restrict__ x, double* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows_total) return;
+  const double* __restrict__ Mr = M + row * (int64_t)m;
+  const double* __restrict__ xb = x + (row / m) * (int64_t)m;
+  double acc0 = 0.0, acc1 = 0.0;
+  if (pairs) {
+    const double2* __restrict__ M2 = reinterpret_cast<const double2*>(Mr);
+    const double2* __restrict__ x2 = reinterpret_cast<const double2*>(xb);
+    const int np = m >> 1;
+    int p = lane;
+    for (; p + 192 < np; p += 256) {
+      const double2 a0 = M2[p], a1 = M2[p + 64], a2 = M2[p + 128], a3 = M2[p + 192];
+      const double2 v0 = x2[p], v1 = x2[p + 64], v2 = x2[p + 128], v3 = x2[p + 192];
+      acc0 = fma(a0.x, v0.x, acc0);
+      acc1 = fma(a0.y, v0.y, acc1);
+      acc0 = fma(a1.x, v1.x, acc0);
+      acc1 = fma(a1.y, v1.y, acc1);
+      acc0 = fma(a2.x, v2.x, acc0);
+      acc1 = fma(a2.y, v2.y, acc1);
+      acc0 = fma(a3.x, v3.x, acc0);
+      acc1 = fma(a3.y, v3.y, acc1);
+    }
+    for (; p < np; p += 64) {
+      const double2 a = M2[p], v = x2[p];
+      acc0 = fma(a.x, v.x, acc0);
+      acc1 = fma(a.y, v.y, acc1);
+    }
+  } else {
+    for (int j = lane; j < m; j += 64) acc0 = fma(Mr[j], xb[j], acc0);
+  }
+  const double sum = wave_sum_f64(acc0 + acc1);
+  if (lane == 0) y[row] = sum;
+}
+
 static int num_cus() {
   static int cached = 0;
   if (cached == 0) {
@@ -830,6 +877,21 @@ extern "C" int f3d_lu_solve_batched(int precision, int64_t nb, int32_t nmax, con
                (const long*)a_off, (const long*)x_off, A, lu32, rs, cs, perm, x, y, refine, tol,
                iters);
   }
+  return F3D_OK;
+}
+
+extern "C" int f3d_dense_gemv_batched(int64_t count, int32_t m, const double* M,
+                                      const double* x, double* y, void* stream) {
+  F3D_REQUIRE(count >= 0 && m >= 0, F3D_EINVAL, "dense_gemv: count=%lld m=%d",
+              (long long)count, m);
+  if (count == 0 || m == 0) return F3D_OK;
+  F3D_REQUIRE(M && x && y, F3D_EINVAL, "dense_gemv: NULL argument");
+  const int64_t rows = count * (int64_t)m;
+  F3D_REQUIRE(rows / 4 < (1ll << 31), F3D_EINVAL, "dense_gemv: %lld rows", (long long)rows);
+  const int pairs = (m % 2 == 0) && (((uintptr_t)M | (uintptr_t)x) % 16 == 0);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  F3D_LAUNCH(dense_gemv_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st, rows, m,
+             pairs, M, x, y);
   return F3D_OK;
 }
 

@@ -135,13 +135,11 @@ class BlockGemv:
         return y
 
 
-class DenseRowBlocks:
+class DenseGemv:
     """y = M x for a stack of dense square FP64 matrices through
-    f3d_block_gemv_batched: each (m x m) matrix is cut into row blocks of at
-    most 256 rows, stored column-major.  Unlike a BLAS GEMV this allocates
-    nothing per call, so it can sit inside a recorded HIP graph."""
-
-    ROWS = 256
+    f3d_dense_gemv_batched (row-major, one wave per row).  Unlike a BLAS GEMV
+    this allocates nothing per call, so it can sit inside a recorded HIP
+    graph."""
 
     def __init__(self, mats):
         """mats: (count, m, m) or (m, m) FP64 CUDA tensor."""
@@ -149,19 +147,15 @@ class DenseRowBlocks:
             mats = mats[None]
         count, m, m2 = mats.shape
         assert m == m2 and mats.dtype == torch.float64
-        r0 = np.arange(0, m, self.ROWS)
-        rows = np.minimum(self.ROWS, m - r0)
         self.count, self.m = count, m
-        ms = np.tile(rows, count)
-        ks = np.full(ms.size, m)
-        x_off = np.repeat(np.arange(count) * m, rows.size)
-        y_off = (np.arange(count)[:, None] * m + r0[None, :]).ravel()
-        self.g = BlockGemv(ms, ks, x_off, y_off)
-        parts = [mats[c, a:a + r].t().reshape(-1) for c in range(count)
-                 for a, r in zip(r0, rows)]
-        self.M = torch.cat(parts).contiguous()
+        self.M = mats.contiguous()
 
     def __call__(self, x, y):
         """x, y: contiguous (count * m,) FP64 tensors (y must not alias x)."""
-        return self.g(self.M, x, y)
+        assert x.numel() == y.numel() == self.count * self.m
+        assert x.is_contiguous() and y.is_contiguous() and x.data_ptr() != y.data_ptr()
+        rc = L.load().f3d_dense_gemv_batched(self.count, self.m, L.ptr(self.M), L.ptr(x),
+                                             L.ptr(y), _stream())
+        L.check(rc, "f3d_dense_gemv_batched")
+        return y
 

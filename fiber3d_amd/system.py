@@ -29,7 +29,7 @@ import torch
 
 from . import _lib as L
 from . import fiber_matrices as FMX
-from .dense import BatchedLU, BlockGemv, BlockSet, DenseRowBlocks
+from .dense import BatchedLU, BlockGemv, BlockSet, DenseGemv
 from .kernels import _stream, device, pairsum, periphery_apply
 
 
@@ -72,7 +72,7 @@ class Shell:
     pc='dense' builds the reference's dense operator (tstep.py:328-345) and
     inverts it once; the reference LU-factors it (tstep.py:346-351) and
     applies lu_solve, the same linear map.  The inverse is applied in FP64 by
-    the libf3d block GEMV: bandwidth-bound instead of a latency-bound pair of
+    the libf3d dense GEMV: bandwidth-bound instead of a latency-bound pair of
     triangular sweeps, and free of BLAS/rocSOLVER calls, whose workspaces a
     HIP graph cannot keep.  pc_f32 is accepted and ignored.  pc='block'
     inverts only each node's 3x3 diagonal block, -E_i/w_i - I/w_i + n_i n_i^T
@@ -102,9 +102,9 @@ class Shell:
         self.pc = pc
         self.minv = None
         if pc == "dense":
-            # FP64 inverse applied with the batched block GEMV (no BLAS call,
+            # FP64 inverse applied with the libf3d dense GEMV (no BLAS call,
             # so the preconditioner can be recorded in a HIP graph)
-            self.minv = DenseRowBlocks(dense_inverse(self.dense_matrix()))
+            self.minv = DenseGemv(dense_inverse(self.dense_matrix()))
         elif pc == "block":
             E = torch.stack(self.e, dim=2)  # E[i, a, c] = e_c[i, a]
             winv = (1.0 / self.w)[:, None, None]
@@ -188,7 +188,7 @@ class Bodies:
         # rotation of every body from the pose its inverse block was built in
         self.rot = torch.eye(3, dtype=torch.float64, device=dev).repeat(self.count, 1, 1)
         self.update_geometry()
-        self.ainv = DenseRowBlocks(torch.stack([dense_inverse(A) for A in self.pc_blocks()]))
+        self.ainv = DenseGemv(torch.stack([dense_inverse(A) for A in self.pc_blocks()]))
         m = 3 * self.nb + 6
         self._xb = torch.empty(self.count * m, dtype=torch.float64, device=dev)
         self._yb = torch.empty(self.count * m, dtype=torch.float64, device=dev)
@@ -297,7 +297,7 @@ class Bodies:
 
     def precond(self, x, out, start):
         """Rotate every 3-vector of a body's slice into the body frame
-        ((R^T v)_c = sum_a R_ac v_a), apply A0^-1 (batched block GEMV),
+        ((R^T v)_c = sum_a R_ac v_a), apply A0^-1 (batched dense GEMV),
         rotate back ((R y)_c = sum_a R_ca y_a).  Elementwise products and the
         libf3d GEMV only, no BLAS, so the step can be recorded in a graph."""
         k = self.nb + 2

@@ -233,6 +233,17 @@ int f3d_block_gemv_batched(int64_t nb, int32_t mmax, const int32_t* m, const int
                            const double* M, const double* x, double* y, double alpha,
                            double beta, void* stream);
 
+/*
+ * y_b = M_b x_b for `count` dense ROW-major m x m FP64 matrices stored
+ * back to back (M + b*m*m), x and y of length count*m (y must not alias x).
+ * Applies the shell preconditioner's dense inverse, the same linear map as the
+ * reference's lu_solve with the shell LU (python/tstep/tstep.py:346-351,
+ * :957-958), and the rigid bodies' precomputed inverses
+ * (python/tstep/tstep.py:225-252, :966-980).
+ */
+int f3d_dense_gemv_batched(int64_t count, int32_t m, const double* M, const double* x,
+                           double* y, void* stream);
+
 /* ---- BF16 MFMA GEMM for the learned initial-guess network ------------ */
 /*
  * No counterpart in the reference (the learned GMRES initial guess is new).
