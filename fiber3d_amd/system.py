@@ -77,10 +77,11 @@ class Shell:
     HIP graph cannot keep.  pc_f32 is accepted and ignored.  pc='block'
     inverts only each node's 3x3 diagonal block, -E_i/w_i - I/w_i + n_i n_i^T
     (0.1 ms per apply at 32768 nodes; about twice the GMRES iterations of the
-    dense form).  The dense inverse grows as (3N)^2: 4.6 GB at 8000 nodes,
-    77 GB at 32768, so pc='auto' picks 'dense' only up to 2048 nodes."""
+    dense form).  The dense inverse grows as (3N)^2: 4.6 GB and 7 s of set-up
+    at 8000 nodes, 77 GB and minutes of set-up at 32768, so pc='auto' picks
+    'dense' up to 8192 nodes."""
 
-    DENSE_MAX = 2048
+    DENSE_MAX = 8192
 
     def __init__(self, x, normals, weights, eta=1.0, reg=5e-3, eps=1e-5, pc="auto",
                  pc_f32=False):
