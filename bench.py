@@ -114,9 +114,10 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
                          epsilon=spec.epsilon, tol=cfg.gmres_tol, restart=cfg.restart,
                          maxiter=cfg.maxiter, predictor=pred, lengths=np.full(len(fibers), spec.length),
                          bodies=bodies, graph=graph, record=mode == "fast")
-        # the learned mode collects samples from its 4th step on and trains
-        # after train_steps of them: all of that happens before timing starts
-        warm = train_steps + 4 if pred is not None else 1
+        # a predictor needs `keep` solutions before its full guess; the learned
+        # mode then collects train_steps samples and trains: all of that
+        # happens before timing starts
+        warm = 1 if pred is None else pred.keep + 2 + (train_steps if pred.use_net else 0)
         for _ in range(warm):
             ts.step()
         ts.records = []
@@ -175,7 +176,7 @@ def small_leg(steps=30, cpu=True):
                          epsilon=spec.epsilon, tol=cfg.gmres_tol, restart=cfg.restart,
                          maxiter=cfg.maxiter, predictor=pred,
                          lengths=np.full(len(fibers), spec.length))
-        for _ in range(4):
+        for _ in range(4 if pred is None else pred.keep + 2):
             ts.step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()

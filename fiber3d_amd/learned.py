@@ -6,13 +6,15 @@ fallback: fiber3d_amd.gmres drops a guess that is not finite or whose residual
 exceeds ||b||, and the FP64 residual alone decides convergence.
 
 Network: 3 layers, width 256, ReLU, applied independently to every fiber node
-(predicting the tension change T_n - T_{n-1}) and every shell node (predicting
-coefficients of the density change in a frame built from local vectors).
-Forward and backward products run on MFMA in BF16 with FP32 accumulation
-(libf3d f3d_gemm_bf16); master weights and the Adam update are FP32.
+(predicting the correction of the extrapolated position in a local frame) and
+every shell node (predicting coefficients of the density correction in a basis
+built from local vectors).  Forward and backward products run on MFMA in BF16
+with FP32 accumulation (libf3d f3d_gemm_bf16); master weights and the Adam
+update are FP32.
 
-Guess assembly per step: the previous two solutions extrapolated linearly,
-corrected by the nets (see LearnedGuess).
+Guess assembly per step: the previous solutions extrapolated (positions by a
+least-squares quadratic in time, the rest linearly), corrected by the nets
+(see LearnedGuess).
 """
 import numpy as np
 import torch
@@ -124,33 +126,55 @@ class Normalizer:
         return ((X - self.mu) / self.sd).contiguous()
 
 
+def lsq_weights(npts, deg):
+    """Weights w_k (k = 0 newest .. npts-1 oldest) that evaluate, one step
+    ahead, the least-squares polynomial of degree `deg` through the last
+    `npts` equally spaced values: x_next = sum_k w_k x_{n-k}."""
+    if not (1 <= deg < npts):
+        raise ValueError("need 1 <= deg < npts")
+    t = -np.arange(npts, dtype=float)
+    A = np.vander(t, deg + 1, increasing=True)
+    return np.ones(deg + 1) @ np.linalg.pinv(A)
+
+
 class LearnedGuess:
     """Initial-guess predictor for TimeStepper (fiber3d_amd.tstep).
 
-    Baseline guess: the solution extrapolated linearly from the last two steps
-    (positions, tensions and shell density alike).  Two MLPs correct it: one
-    per fiber node predicts the tension correction T_n - T_ext, one per shell
-    node predicts coefficients c_k of the density correction q_n - q_ext =
-    sum_k c_k u_k in a local basis u = (dq, d2q, n (n . dq)) built from the
-    density history.  Inputs are rotation-invariant scalars of the local
-    geometry, the explicit flow and the solution history.  The last layer
-    starts at zero, so an untrained net reproduces the extrapolation, and after
-    training each net is kept only if it beats the extrapolation on held-out
-    steps (validation MSE below `gate` times the extrapolation's).
+    Baseline guess: every component extrapolated linearly from the last two
+    solutions, except the fiber positions X, which come from the least-squares
+    polynomial of degree fit[1] through the last fit[0] positions.  Nearly all
+    of the guess's residual is in X: putting the converged X into the guess
+    leaves no GMRES iteration to do, while the converged tensions or shell
+    densities barely change the count.  The linear extrapolation of X
+    carries a second difference dt^2 X'' that stays nearly constant from step
+    to step.  A quadratic through three points removes it but multiplies the
+    solves' own errors by up to 7.  The least-squares quadratic through six
+    points removes it and multiplies them by at most 3.6.
+
+    Two MLPs correct the baseline.  One per fiber node predicts the position
+    correction X_n - X_fit as its components in an orthonormal local frame
+    (tangent, the part of the last displacement normal to it, their cross
+    product).  One per shell node predicts coefficients c_k of the density
+    correction q_n - q_ext = sum_k c_k u_k in a local basis u = (dq, d2q,
+    n (n . dq)) built from the density history.  Inputs are rotation-invariant
+    scalars of the local geometry, the explicit flow and the solution history.
+    The last layer starts at zero, so an untrained net reproduces the
+    baseline.  After training, each net is kept only if it beats the baseline
+    on held-out steps (validation MSE below `gate` times the baseline's).
     """
 
     def __init__(self, train_steps=100, epochs=200, lr=1e-3, batch=8192, seed=0,
-                 use_net=True, gate=0.9, val_frac=0.2, order=1):
-        if order not in (1, 2):
-            raise ValueError("extrapolation order is 1 or 2")
-        self.order = order
+                 use_net=True, gate=0.9, val_frac=0.2, fit=(6, 2)):
+        self.fit = (int(fit[0]), int(fit[1]))
+        self.wx = lsq_weights(*self.fit)
+        self.keep = max(3, self.fit[0])
         self.train_steps = train_steps
         self.epochs, self.lr, self.batch, self.seed = epochs, lr, batch, seed
         self.use_net = use_net
         self.gate, self.val_frac = gate, val_frac
         self.samples_f = []
         self.samples_s = []
-        self.prev = []  # last three solutions
+        self.prev = []  # last `keep` solutions, oldest first
         self.net_f = self.net_s = None
         self.enabled_f = self.enabled_s = False
         self.trained = False
@@ -158,29 +182,44 @@ class LearnedGuess:
 
     # ---- features ---------------------------------------------------------
     def _hist(self, ts, idx):
-        """T (or any gathered component) at the last three solutions."""
-        return [p[idx] for p in self.prev[-3:]]
+        """Any gathered component at the last three solutions (newest first)."""
+        return [self.prev[-1 - k][idx] for k in range(3)]
 
-    def _fiber_features(self, ts):
+    def _fiber_terms(self, ts):
+        """Per fiber node: 15 rotation-invariant features and the orthonormal
+        frame (t, e2, e3) the position correction is expressed in."""
         S = ts.sys
-        v = S.v_explicit
-        T1, T2, T3 = self._hist(ts, ts.idx_t)
-        out = torch.empty((S.npts, 10), dtype=torch.float64, device=S.dev)
-        for N, (node, X, xs, xss, D1, P, B) in S._ctx.items():
-            vk = v[node]
-            nf = node.reshape(-1)
-            alpha = S.buckets[N]["alpha"][None, :].expand(B, N)
-            dv = torch.matmul(D1, vk)
-            t1, t2, t3 = T1[node], T2[node], T3[node]
-            f = torch.stack([alpha, alpha.abs(), t1, t1 - t2, t1 - 2 * t2 + t3,
-                             (xs * dv).sum(-1), (xss * vk).sum(-1), (xss * xss).sum(-1),
-                             (xs * vk).sum(-1), vk.norm(dim=-1)], dim=-1)
-            out[nf] = f.reshape(-1, 10)
-        return out
+        X1, X2, X3 = (p.view(-1, 3) for p in self._hist(ts, ts.idx_x))
+        T1, T2, _ = self._hist(ts, ts.idx_t)
+        d1 = X1 - X2
+        d2 = X1 - 2.0 * X2 + X3
+        t = S.xs / torch.linalg.norm(S.xs, dim=1, keepdim=True).clamp_min(1e-300)
+        p = d1 - (d1 * t).sum(1, keepdim=True) * t
+        pn = torch.linalg.norm(p, dim=1, keepdim=True)
+        # a displacement along the tangent leaves the normal direction free:
+        # take the unit axis least aligned with t, made orthogonal to it
+        ax = torch.zeros_like(t)
+        ax.scatter_(1, t.abs().argmin(1, keepdim=True), 1.0)
+        q = ax - (ax * t).sum(1, keepdim=True) * t
+        use_p = pn > 1e-12 * torch.linalg.norm(d1, dim=1, keepdim=True).clamp_min(1e-300)
+        e2 = torch.where(use_p, p, q)
+        e2 = e2 / torch.linalg.norm(e2, dim=1, keepdim=True).clamp_min(1e-300)
+        e3 = torch.cross(t, e2, dim=1)
+        frame = torch.stack([t, e2, e3], dim=1)  # (npts, 3, 3), rows are the axes
+        comp = lambda v: torch.einsum("nij,nj->ni", frame, v)
+        alpha = torch.empty(S.npts, dtype=torch.float64, device=S.dev)
+        xss = torch.empty_like(S.xs)
+        for N, (node, X, xs, xss_k, D1, P, B) in S._ctx.items():
+            alpha[node.reshape(-1)] = S.buckets[N]["alpha"][None, :].expand(B, N).reshape(-1)
+            xss[node.reshape(-1)] = xss_k.reshape(-1, 3)
+        c1, c2, cv, cs = comp(d1), comp(d2), comp(S.v_explicit), comp(xss)
+        feats = torch.cat([alpha[:, None], alpha.abs()[:, None], c1[:, :2], c2, cv, cs,
+                           T1[:, None], (T1 - T2)[:, None]], dim=1)
+        return feats, frame
 
     def _shell_terms(self, ts):
         S = ts.sys
-        q1, q2, q3 = (p[:3 * S.nper].view(S.nper, 3) for p in self.prev[-3:])
+        q1, q2, q3 = (p[:3 * S.nper].view(S.nper, 3) for p in self._hist(ts, slice(None)))
         dq = q1 - q2
         d2q = dq - (q2 - q3)
         n = S.shell.nrm
@@ -192,12 +231,17 @@ class LearnedGuess:
                            (n * v).sum(-1)], dim=-1)
         return u, inv
 
-    def _extrapolated(self):
-        """Linear (2 p1 - p2) or, with order=2 and three solutions on hand,
-        quadratic (3 p1 - 3 p2 + p3) extrapolation in time."""
-        if self.order == 2 and len(self.prev) >= 3:
-            return 3.0 * (self.prev[-1] - self.prev[-2]) + self.prev[-3]
-        return 2.0 * self.prev[-1] - self.prev[-2]
+    def _extrapolated(self, ts):
+        """Linear extrapolation 2 p1 - p2 of every component; fiber positions
+        from the least-squares polynomial fit once fit[0] solutions exist."""
+        x0 = 2.0 * self.prev[-1] - self.prev[-2]
+        if len(self.prev) >= len(self.wx):
+            ix = ts.idx_x.reshape(-1)
+            xf = torch.zeros(ix.numel(), dtype=x0.dtype, device=x0.device)
+            for k, w in enumerate(self.wx):
+                xf.add_(self.prev[-1 - k][ix], alpha=float(w))
+            x0[ix] = xf
+        return x0
 
     # ---- predictor protocol ------------------------------------------------
     def predict(self, ts):
@@ -205,33 +249,35 @@ class LearnedGuess:
         self._pending = None
         if len(self.prev) < 2:
             return self.prev[-1].clone() if self.prev else None
-        x0 = self._extrapolated()
-        if len(self.prev) < 3:
+        x0 = self._extrapolated(ts)
+        if len(self.prev) < self.keep or not self.use_net:
             return x0
-        ff = self._fiber_features(ts)
-        sh = self._shell_terms(ts) if S.nper else None
         if not self.trained:
-            self._pending = (ff, sh, x0)
-        elif self.use_net:
-            with torch.no_grad():
-                if self.enabled_f:
-                    dT = self.net_f.forward(self.nf(ff.float()))[:, 0].double() * self.t_scale
-                    x0[ts.idx_t] += dT
-                if sh is not None and self.enabled_s:
-                    u, inv = sh
-                    c = self.net_s.forward(self.ns(inv.float())).double() * self.c_scale
-                    x0[:3 * S.nper] += (u * c[:, :, None]).sum(1).reshape(-1)
+            sh = self._shell_terms(ts) if S.nper else None
+            self._pending = (*self._fiber_terms(ts), sh, x0)
+            return x0
+        # trained: only the nets the gate kept are evaluated
+        with torch.no_grad():
+            if self.enabled_f:
+                ff, frame = self._fiber_terms(ts)
+                c = self.net_f.forward(self.nf(ff.float())).double() * self.x_scale
+                ix = ts.idx_x
+                x0[ix] = x0[ix] + torch.einsum("nij,ni->nj", frame, c)
+            if S.nper and self.enabled_s:
+                u, inv = self._shell_terms(ts)
+                c = self.net_s.forward(self.ns(inv.float())).double() * self.c_scale
+                x0[:3 * S.nper] += (u * c[:, :, None]).sum(1).reshape(-1)
         return x0
 
     def observe(self, ts, sol):
         S = ts.sys
         pend = getattr(self, "_pending", None)
-        self.prev = (self.prev + [sol.clone()])[-3:]
+        self.prev = (self.prev + [sol.clone()])[-self.keep:]
         if self.trained or pend is None:
             return
-        ff, sh, x_ext = pend
-        dT = (sol - x_ext)[ts.idx_t]
-        self.samples_f.append((ff, dT[:, None]))
+        ff, frame, sh, x_ext = pend
+        dX = (sol - x_ext)[ts.idx_x]  # (npts, 3)
+        self.samples_f.append((ff, torch.einsum("nij,nj->ni", frame, dX)))
         if sh is not None:
             u, inv = sh
             dq = (sol - x_ext)[:3 * S.nper].view(S.nper, 3)
@@ -256,7 +302,8 @@ class LearnedGuess:
         Xt, Yt, Xv, Yv = self._split(samples)
         norm = Normalizer()
         norm.fit(Xt)
-        scale = float(Yt.abs().max().clamp_min(1e-30))
+        # per-output scale: the outputs may differ by orders of magnitude
+        scale = Yt.abs().amax(0).clamp_min(1e-30)
         net = MLP(Xt.shape[1], n_out, seed=seed)
         net.W[2].zero_()
         self._fit(net, norm(Xt), (Yt / scale).contiguous())
@@ -268,9 +315,9 @@ class LearnedGuess:
 
     def train(self):
         torch.manual_seed(self.seed)
-        self.net_f, self.nf, self.t_scale, mn, mb = self._train_one(self.samples_f, 1, self.seed)
+        self.net_f, self.nf, self.x_scale, mn, mb = self._train_one(self.samples_f, 3, self.seed)
         self.enabled_f = mn < self.gate * mb
-        self.report["fiber"] = {"val_mse_net": mn, "val_mse_extrapolation": mb,
+        self.report["fiber"] = {"val_mse_net(position)": mn, "val_mse_fit(position)": mb,
                                 "enabled": self.enabled_f}
         if self.samples_s:
             self.net_s, self.ns, self.c_scale, mn, mb = self._train_one(self.samples_s, 3,

@@ -47,7 +47,7 @@ def test_learned_guess_reduces_iterations():
                        ("learned", LearnedGuess(train_steps=train, epochs=60))):
         ts = TimeStepper(fibers, shell, mode="fast", tol=1e-8, predictor=pred)
         its = []
-        for _ in range(train + 12):
+        for _ in range(train + 16):
             st = ts.step()
             assert st.residual <= 1e-8
             its.append(st.iterations)
