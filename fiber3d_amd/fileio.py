@@ -1,0 +1,108 @@
+"""Fiber files in the reference's text formats.
+
+Reading follows read_input/read_fibers_file.py:19-68: '#' starts a comment,
+blank lines are skipped, the first value is the number of fibers, and each
+fiber is a header line "num_points [E [length [growing]]]" followed by
+num_points lines "x y z".
+
+Writing follows tstep.write_to_file (python/tstep/tstep.py:2158-2177):
+  * a trajectory frame of <output>_<ID>_fibers.txt (output_txt_files=True,
+    the reference default): a row of four copies of the fiber count, then per
+    fiber the row (num_points, E, length, growing) and num_points rows
+    (x, y, z, tension), every row written by numpy.savetxt's default '%.18e';
+  * the same frame for <output>_<ID>.fibers (output_txt_files=False): the
+    fiber count and the per-fiber header as plain '%s' text, the rows by
+    numpy.savetxt;
+  * <output>_<ID>_resume.fibers: the fiber count, then per fiber the '%s'
+    header and num_points rows (x, y, z).
+Output is byte-for-byte what the reference writes for the same values.
+"""
+import numpy as np
+
+
+def read_fibers_file(name_file):
+    """(info, coor): info[k] = [num_points, (E), (length), (growing)] as
+    given in fiber k's header; coor = (sum num_points, 3) float64 array."""
+    info, coor = [], []
+    remaining = 0
+    nfib = None
+    with open(name_file, "r") as f:
+        for line in f:
+            line = line.split("#", 1)[0].strip()
+            if not line:
+                continue
+            data = line.split()
+            if nfib is None:
+                nfib = int(float(data[0]))
+                continue
+            if remaining == 0:
+                head = [int(float(data[0]))]
+                if len(data) > 1:
+                    head.append(float(data[1]))
+                if len(data) > 2:
+                    head.append(float(data[2]))
+                if len(data) > 3:
+                    # the reference converts a textual flag with bool(): any
+                    # non-empty string, "0" included, reads as True
+                    head.append(bool(data[3]))
+                info.append(head)
+                remaining = head[0]
+            else:
+                coor.append([float(data[0]), float(data[1]), float(data[2])])
+                remaining -= 1
+    return info, np.array(coor)
+
+
+def fibers_from_file(name_file, E=None, length=None):
+    """Per-fiber position arrays plus per-fiber E and length (header values
+    where given, else the defaults passed here)."""
+    info, coor = read_fibers_file(name_file)
+    xs, Es, Ls = [], [], []
+    o = 0
+    for head in info:
+        n = head[0]
+        xs.append(coor[o:o + n].copy())
+        Es.append(head[1] if len(head) > 1 else E)
+        Ls.append(head[2] if len(head) > 2 else length)
+        o += n
+    return xs, Es, Ls
+
+
+def _rows(f, a):
+    np.savetxt(f, np.asarray(a, dtype=np.float64))
+
+
+def write_fibers_frame(f, positions, tensions, E, lengths, growing=None, txt=True):
+    """One output frame for the fibers of one type.  f: a file opened in
+    binary mode; positions[k] (N_k, 3); tensions[k] (N_k,); E, lengths,
+    growing: per fiber (scalars are broadcast)."""
+    n = len(positions)
+    E = np.broadcast_to(np.asarray(E, dtype=object), (n,))
+    lengths = np.broadcast_to(np.asarray(lengths, dtype=object), (n,))
+    growing = np.broadcast_to(np.asarray(0 if growing is None else growing, dtype=object), (n,))
+    if txt:
+        np.savetxt(f, np.ones((1, 4), dtype=int) * n)
+    else:
+        f.write((str(n) + "\n").encode("utf-8"))
+    for k in range(n):
+        x = np.asarray(positions[k], dtype=np.float64)
+        if txt:
+            head = np.array([x.shape[0], E[k], lengths[k], growing[k]], dtype=np.float64)
+            np.savetxt(f, head[None, :])
+        else:
+            f.write(("%s %s %s %s \n" % (x.shape[0], E[k], lengths[k], growing[k])).encode("utf-8"))
+        _rows(f, np.concatenate((x, np.asarray(tensions[k], np.float64)[:, None]), axis=1))
+
+
+def write_resume_fibers(name_file, positions, E, lengths, growing=None):
+    """<output>_<ID>_resume.fibers for the fibers of one type."""
+    n = len(positions)
+    E = np.broadcast_to(np.asarray(E, dtype=object), (n,))
+    lengths = np.broadcast_to(np.asarray(lengths, dtype=object), (n,))
+    growing = np.broadcast_to(np.asarray(0 if growing is None else growing, dtype=object), (n,))
+    with open(name_file, "wb") as f:
+        f.write(("%s \n" % n).encode("utf-8"))
+        for k in range(n):
+            x = np.asarray(positions[k], dtype=np.float64)
+            f.write(("%s %s %s %s \n" % (x.shape[0], E[k], lengths[k], growing[k])).encode("utf-8"))
+            _rows(f, x)

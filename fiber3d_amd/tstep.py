@@ -29,6 +29,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
+from .fileio import write_fibers_frame, write_resume_fibers
 from .gmres import gmres
 from .graphs import ArnoldiGraphs
 from .system import FiberSystem
@@ -147,6 +148,27 @@ class TimeStepper:
                        res.guess_used, res.residual_kind, res.residual_bound)
         self.history.append(st)
         return st
+
+    def fiber_arrays(self):
+        """Host copies of every fiber's positions (N_k, 3) and tensions (N_k,)."""
+        S = self.sys
+        X = S.X.cpu().numpy()
+        T = self.tension.cpu().numpy()
+        o = S.offs
+        return ([X[o[k]:o[k + 1]] for k in range(S.nfib)],
+                [T[o[k]:o[k + 1]] for k in range(S.nfib)])
+
+    def write_frame(self, f, txt=True):
+        """Append the current fibers as one frame of the reference's
+        <output>_<ID>_fibers.txt (txt=True) or <output>_<ID>.fibers output
+        (tstep.py:2158-2177); f is opened in binary mode."""
+        X, T = self.fiber_arrays()
+        write_fibers_frame(f, X, T, self.sys.E, self.sys.lengths_h, txt=txt)
+
+    def write_resume(self, name_file):
+        """The reference's <output>_<ID>_resume.fibers for the current fibers."""
+        X, _ = self.fiber_arrays()
+        write_resume_fibers(name_file, X, self.sys.E, self.sys.lengths_h)
 
     def verify(self):
         """True relative residuals ||b - A x|| / ||b|| of the recorded steps in

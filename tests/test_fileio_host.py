@@ -1,0 +1,70 @@
+"""Reference-format fiber files: reading the reference's own example
+structures (tests/golden/*.fibers, copied from python/multi_fibers/), and
+writing trajectory frames and resume files that read back exactly."""
+import io
+import os
+
+import numpy as np
+
+from fiber3d_amd.fileio import (fibers_from_file, read_fibers_file, write_fibers_frame,
+                                write_resume_fibers)
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_read_reference_example_two_fibers():
+    info, coor = read_fibers_file(os.path.join(GOLDEN, "two.fibers"))
+    assert info == [[32, 1.0, 1.85], [32, 1.0, 1.85]]
+    assert coor.shape == (64, 3)
+    # both fibers lie on the z axis, the second shifted by 2 (file values)
+    assert np.all(coor[:, :2] == 0.0)
+    assert coor[0, 2] == 2.0 and coor[32, 2] == 4.0
+    assert coor[31, 2] == 0.1499999999999999944
+
+
+def test_read_skips_comments_and_blank_lines():
+    info, coor = read_fibers_file(os.path.join(GOLDEN, "one.fibers"))
+    assert info == [[32, 1.0, 1.0]] and coor.shape == (32, 3)
+    xs, Es, Ls = fibers_from_file(os.path.join(GOLDEN, "one.fibers"))
+    assert len(xs) == 1 and xs[0].shape == (32, 3) and Es == [1.0] and Ls == [1.0]
+
+
+def test_growing_flag_reads_like_the_reference(tmp_path):
+    """The reference converts the textual fourth header field with bool():
+    '0' is a non-empty string and reads as True."""
+    path = tmp_path / "growing.fibers"
+    path.write_text("1\n2 0.01 1.5 0\n0 0 0\n0 0 1\n")
+    info, coor = read_fibers_file(path)
+    assert info == [[2, 0.01, 1.5, True]] and coor.shape == (2, 3)
+
+
+def test_resume_file_round_trip(tmp_path):
+    rng = np.random.default_rng(0)
+    pos = [rng.normal(size=(n, 3)) for n in (16, 24, 32)]
+    name = tmp_path / "run_fibers_resume.fibers"
+    write_resume_fibers(name, pos, 0.01, [1.0, 1.5, 2.0])
+    info, coor = read_fibers_file(name)
+    assert info == [[16, 0.01, 1.0, True], [24, 0.01, 1.5, True], [32, 0.01, 2.0, True]]
+    assert np.array_equal(coor, np.concatenate(pos))   # '%.18e' round-trips FP64 exactly
+    text = name.read_text().splitlines()
+    assert text[0] == "3 " and text[1] == "16 0.01 1.0 0 "
+
+
+def test_trajectory_frame_layouts():
+    pos = [np.arange(6.0).reshape(2, 3), np.arange(9.0).reshape(3, 3) + 0.5]
+    ten = [np.array([1.0, 2.0]), np.array([3.0, 4.0, 5.0])]
+    buf = io.BytesIO()
+    write_fibers_frame(buf, pos, ten, 0.01, [1.0, 2.0], txt=True)
+    rows = np.loadtxt(io.StringIO(buf.getvalue().decode()))
+    assert rows.shape == (1 + 1 + 2 + 1 + 3, 4)
+    assert np.all(rows[0] == 2)
+    assert np.array_equal(rows[1], [2, 0.01, 1.0, 0]) and np.array_equal(rows[4], [3, 0.01, 2.0, 0])
+    assert np.array_equal(rows[2:4], np.hstack([pos[0], ten[0][:, None]]))
+    assert np.array_equal(rows[5:], np.hstack([pos[1], ten[1][:, None]]))
+    first = buf.getvalue().decode().splitlines()[0]
+    assert first == " ".join(["2.000000000000000000e+00"] * 4)
+    buf = io.BytesIO()
+    write_fibers_frame(buf, pos, ten, 0.01, [1.0, 2.0], txt=False)
+    lines = buf.getvalue().decode().splitlines()
+    assert lines[0] == "2" and lines[1] == "2 0.01 1.0 0 "
+    assert len(lines) == 1 + (1 + 2) + (1 + 3)
