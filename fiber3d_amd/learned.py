@@ -126,15 +126,30 @@ class Normalizer:
         return ((X - self.mu) / self.sd).contiguous()
 
 
+def lsq_weights_at(times, t_next, deg):
+    """Weights w_k over values at `times` (newest first) that evaluate, at
+    t_next, the least-squares polynomial of degree `deg` through them:
+    x(t_next) ~ sum_k w_k x(times[k]).  Times are measured in units of the
+    newest interval, which keeps the Vandermonde matrix well conditioned."""
+    t = np.asarray(times, dtype=float)
+    if not (1 <= deg < t.size):
+        raise ValueError("need 1 <= deg < len(times)")
+    h = t[0] - t[1]
+    if not h > 0:
+        raise ValueError("times must decrease from newest to oldest")
+    tau = (t - t[0]) / h
+    A = np.vander(tau, deg + 1, increasing=True)
+    e = ((t_next - t[0]) / h) ** np.arange(deg + 1)
+    return e @ np.linalg.pinv(A)
+
+
 def lsq_weights(npts, deg):
     """Weights w_k (k = 0 newest .. npts-1 oldest) that evaluate, one step
     ahead, the least-squares polynomial of degree `deg` through the last
     `npts` equally spaced values: x_next = sum_k w_k x_{n-k}."""
     if not (1 <= deg < npts):
         raise ValueError("need 1 <= deg < npts")
-    t = -np.arange(npts, dtype=float)
-    A = np.vander(t, deg + 1, increasing=True)
-    return np.ones(deg + 1) @ np.linalg.pinv(A)
+    return lsq_weights_at(-np.arange(npts, dtype=float), 1.0, deg)
 
 
 class LearnedGuess:
@@ -166,7 +181,7 @@ class LearnedGuess:
     def __init__(self, train_steps=100, epochs=200, lr=1e-3, batch=8192, seed=0,
                  use_net=True, gate=0.9, val_frac=0.2, fit=(6, 2)):
         self.fit = (int(fit[0]), int(fit[1]))
-        self.wx = lsq_weights(*self.fit)
+        lsq_weights(*self.fit)  # validates the shape
         self.keep = max(3, self.fit[0])
         self.train_steps = train_steps
         self.epochs, self.lr, self.batch, self.seed = epochs, lr, batch, seed
@@ -174,7 +189,8 @@ class LearnedGuess:
         self.gate, self.val_frac = gate, val_frac
         self.samples_f = []
         self.samples_s = []
-        self.prev = []  # last `keep` solutions, oldest first
+        self.prev = []   # last `keep` solutions, oldest first
+        self.times = []  # their times
         self.net_f = self.net_s = None
         self.enabled_f = self.enabled_s = False
         self.trained = False
@@ -232,14 +248,22 @@ class LearnedGuess:
         return u, inv
 
     def _extrapolated(self, ts):
-        """Linear extrapolation 2 p1 - p2 of every component; fiber positions
-        from the least-squares polynomial fit once fit[0] solutions exist."""
-        x0 = 2.0 * self.prev[-1] - self.prev[-2]
-        if len(self.prev) >= len(self.wx):
+        """Linear extrapolation in time of every component to the time of the
+        coming solution (2 p1 - p2 at equal steps); fiber positions from the
+        least-squares polynomial fit once fit[0] solutions exist.  The
+        weights use the solutions' actual times, so steps of changing size
+        (adaptive stepping) are extrapolated correctly."""
+        t_next = ts.time + ts.sys.dt
+        t1, t2 = self.times[-1], self.times[-2]
+        a = (t_next - t1) / (t1 - t2)
+        x0 = (1.0 + a) * self.prev[-1] - a * self.prev[-2]
+        npts, deg = self.fit
+        if len(self.prev) >= npts:
+            w = lsq_weights_at(self.times[::-1][:npts], t_next, deg)
             ix = ts.idx_x.reshape(-1)
             xf = torch.zeros(ix.numel(), dtype=x0.dtype, device=x0.device)
-            for k, w in enumerate(self.wx):
-                xf.add_(self.prev[-1 - k][ix], alpha=float(w))
+            for k, wk in enumerate(w):
+                xf.add_(self.prev[-1 - k][ix], alpha=float(wk))
             x0[ix] = xf
         return x0
 
@@ -273,6 +297,7 @@ class LearnedGuess:
         S = ts.sys
         pend = getattr(self, "_pending", None)
         self.prev = (self.prev + [sol.clone()])[-self.keep:]
+        self.times = (self.times + [float(ts.time)])[-self.keep:]
         if self.trained or pend is None:
             return
         ff, frame, sh, x_ext = pend

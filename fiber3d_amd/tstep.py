@@ -15,6 +15,15 @@ Two modes:
              FP64 operator deciding convergence, FP32 fiber LU with FP64
              refinement, and an optional initial-guess predictor.
 
+Time steps are fixed by default.  adaptive=True follows the reference's
+accept/reject scheme (tstep.py:427-581, check_error :1874-1925): after each
+solve the fibers' inextensibility error max |(|x_s| - 1)| of the new
+configuration decides; a step within tol_tstep is accepted (and the next dt
+grows by 1.2, up to dt_max, when the error is below 0.9 tol_tstep), a step
+above it, or one that puts a fiber node on or outside a spherical periphery
+of radius periphery_radius, is rejected and retried with dt / 2 (down to
+dt_min, where it is accepted).
+
 In fast mode the closing residual is, by default (certify="update"), the
 guess's FP64 residual updated by the correction's image under the
 FP32-compensated operator, accepted only under a bound that includes that
@@ -44,6 +53,23 @@ class StepStats:
     guess_used: bool
     residual_kind: str = "fp64"
     residual_bound: float = 0.0
+    accepted: bool = True
+    dt: float = 0.0                  # the step size this solve used
+    inextensibility: float = 0.0     # max | |x_s| - 1 | of the new configuration
+
+
+def adaptive_decision(err, collision, dt, tol_tstep, dt_min, dt_max):
+    """(accept, next dt) of tstep.check_error (tstep.py:1900-1922)."""
+    if err <= tol_tstep:
+        accept = True
+        dt_new = min(dt_max, dt * 1.2) if err <= 0.9 * tol_tstep else dt
+    else:
+        accept, dt_new = False, dt * 0.5
+    if collision:
+        accept, dt_new = False, dt * 0.5
+    if dt_new < dt_min:
+        accept, dt_new = True, dt_min
+    return accept, dt_new
 
 
 class TimeStepper:
@@ -51,7 +77,8 @@ class TimeStepper:
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference", tol=1e-8,
                  restart=150, maxiter=200, predictor=None, lengths=None, bodies=None,
                  body_force=None, graph=False, certify="update", record=False, lu_every=None,
-                 target=None):
+                 target=None, adaptive=False, tol_tstep=1e-2, dt_min=1e-5, dt_max=1e-2,
+                 periphery_radius=None):
         if mode not in ("plain", "fast"):
             raise ValueError("mode is 'plain' or 'fast'")
         if certify not in ("update", "fp64"):
@@ -65,6 +92,11 @@ class TimeStepper:
         self.lu_every = max(1, int(lu_every if lu_every is not None
                                    else (8 if mode == "fast" else 1)))
         self.target = target
+        self.adaptive = adaptive
+        self.tol_tstep, self.dt_min, self.dt_max = tol_tstep, dt_min, dt_max
+        self.periphery_radius = periphery_radius
+        self.time = 0.0          # time of the last accepted solution
+        self.rejected = 0
         self.nsteps = 0
         self.tol, self.restart, self.maxiter = tol, restart, maxiter
         fast = mode == "fast"
@@ -108,10 +140,23 @@ class TimeStepper:
             return lambda v, out=None: S.matvec(v, out=out, precision="f32c")
         return None
 
+    def inextensibility_error(self):
+        """max over all fiber nodes of | |x_s| - 1 | for the assembled
+        configuration (tstep.check_error, tstep.py:1874-1891)."""
+        return float((torch.linalg.norm(self.sys.xs, dim=1) - 1.0).abs().max())
+
+    def collision(self):
+        """A fiber node on or outside the spherical periphery (the sphere case
+        of tstep.check_collision, tstep.py:1927-1975)."""
+        if self.periphery_radius is None:
+            return False
+        return bool((torch.linalg.norm(self.sys.X, dim=1) >= self.periphery_radius).any())
+
     def step(self):
         S = self.sys
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        dt = S.dt
         x0 = None
         if self.predictor is not None:
             x0 = self.predictor.predict(self)
@@ -126,26 +171,43 @@ class TimeStepper:
                     update_residual=self.mode == "fast" and self.certify == "update",
                     target=self.target, **fused)
         sol = res.x
+        body_old = S.bodies.state() if S.bodies is not None else None
         if self.record:
             # S.rhs and sol are fresh tensors of this step; X is updated in place
-            self.records.append((S.X.clone(), S.bodies.state() if S.bodies is not None else None,
-                                 S.rhs, sol))
-        if self.predictor is not None:
-            self.predictor.observe(self, sol)
-        self.prev_solution = sol
-        self.shell_density = sol[:3 * S.nper].clone()
-        self.tension = sol[self.idx_t]
+            self.records.append((S.X.clone(), body_old, S.rhs, sol, dt))
+        X_old = S.X.clone() if self.adaptive else None
         S.X.copy_(sol[self.idx_x])
         if S.bodies is not None:
             # rigid motion with the solved velocities (tstep.py:1398-1406)
-            S.bodies.move(S.bodies._vel(sol, 3 * S.nper), S.dt)
+            S.bodies.move(S.bodies._vel(sol, 3 * S.nper), dt)
         # re-assembly refills the system's buffers in place: the recorded
         # Arnoldi graphs stay valid
         self.nsteps += 1
         S.prepare(refactor=self.nsteps % self.lu_every == 0)
+        err = self.inextensibility_error()
+        accept, dt_new = True, dt
+        if self.adaptive:
+            accept, dt_new = adaptive_decision(err, self.collision(), dt, self.tol_tstep,
+                                               self.dt_min, self.dt_max)
+        if accept:
+            self.time += dt
+            if self.predictor is not None:
+                self.predictor.observe(self, sol)
+            self.prev_solution = sol
+            self.shell_density = sol[:3 * S.nper].clone()
+            self.tension = sol[self.idx_t]
+        else:
+            # rejected (tstep.py:540-579): back to the configuration it left
+            self.rejected += 1
+            S.X.copy_(X_old)
+            if S.bodies is not None:
+                S.bodies.set_state(body_old)
+        if not accept or dt_new != dt:
+            S.dt = dt_new
+            S.prepare(refactor=True)
         torch.cuda.synchronize()
         st = StepStats(res.iterations, res.restarts, res.residual, time.perf_counter() - t0,
-                       res.guess_used, res.residual_kind, res.residual_bound)
+                       res.guess_used, res.residual_kind, res.residual_bound, accept, dt, err)
         self.history.append(st)
         return st
 
@@ -176,12 +238,13 @@ class TimeStepper:
         the largest difference between the re-assembled and the recorded
         right-hand sides.  The current state is restored afterwards."""
         S = self.sys
-        now = (S.X.clone(), S.bodies.state() if S.bodies is not None else None)
+        now = (S.X.clone(), S.bodies.state() if S.bodies is not None else None, S.dt)
         out, drhs = [], 0.0
-        for X, body, b, x in self.records:
+        for X, body, b, x, dt in self.records:
             S.X.copy_(X)
             if body is not None:
                 S.bodies.set_state(body)
+            S.dt = dt
             S.prepare()
             drhs = max(drhs, float((S.rhs - b).abs().max()))
             r = b - S.matvec(x, precision="f64")
@@ -189,5 +252,6 @@ class TimeStepper:
         S.X.copy_(now[0])
         if now[1] is not None:
             S.bodies.set_state(now[1])
+        S.dt = now[2]
         S.prepare()
         return out, drhs

@@ -32,3 +32,28 @@ def test_lsq_weights_reject_bad_shapes():
         lsq_weights(3, 3)
     with pytest.raises(ValueError):
         lsq_weights(4, 0)
+
+
+def test_lsq_weights_at_uneven_times():
+    """Exact for quadratics at uneven times; equal to the uniform weights at
+    uniform times."""
+    from fiber3d_amd.learned import lsq_weights_at
+    times = np.array([1.0, 0.9, 0.85, 0.7, 0.6, 0.45])   # newest first, uneven steps
+    c = np.array([0.3, -1.2, 2.5])
+    p = lambda t: c[0] + c[1] * t + c[2] * t * t
+    w = lsq_weights_at(times, 1.07, 2)
+    assert abs(w @ p(times) - p(1.07)) < 1e-12
+    u = lsq_weights_at(0.5 - 0.01 * np.arange(6), 0.51, 2)
+    assert np.allclose(u, lsq_weights(6, 2), atol=1e-12)
+
+
+def test_adaptive_decision_rules():
+    """tstep.check_error's accept/reject and step-size rules."""
+    from fiber3d_amd.tstep import adaptive_decision
+    tol, lo, hi = 1e-2, 1e-5, 1e-2
+    assert adaptive_decision(0.5e-2, False, 1e-3, tol, lo, hi) == (True, 1.2e-3)
+    assert adaptive_decision(0.95e-2, False, 1e-3, tol, lo, hi) == (True, 1e-3)
+    assert adaptive_decision(2e-2, False, 1e-3, tol, lo, hi) == (False, 0.5e-3)
+    assert adaptive_decision(0.5e-2, True, 1e-3, tol, lo, hi) == (False, 0.5e-3)
+    assert adaptive_decision(0.1e-2, False, 9e-3, tol, lo, hi) == (True, 1e-2)     # capped at dt_max
+    assert adaptive_decision(2e-2, False, 1.5e-5, tol, lo, hi) == (True, 1e-5)     # floor: accepted
