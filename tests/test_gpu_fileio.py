@@ -19,8 +19,11 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 def test_step_reference_structure_and_write_outputs(tmp_path):
     print("case fileio two.fibers", flush=True)
     xs, Es, Ls = fibers_from_file(os.path.join(GOLDEN, "two.fibers"))
-    xsh, nsh, wsh = configs.make_periphery(600, 5.0)
-    ts = TimeStepper(xs, Shell(xsh, nsh, wsh), mode="plain", E=Es[0], lengths=Ls, tol=1e-10)
+    # the structure spans z in [0.15, 5.85]: a periphery of radius 8 holds it
+    xsh, nsh, wsh = configs.make_periphery(800, 8.0)
+    # the example's nodes are Chebyshev-Lobatto points along each fiber
+    ts = TimeStepper(xs, Shell(xsh, nsh, wsh), mode="plain", E=Es[0], lengths=Ls, tol=1e-10,
+                     discretisation="chebyshev")
     st = ts.step()
     torch.cuda.synchronize()
     assert st.residual <= 1e-10
