@@ -1,7 +1,6 @@
 """The fast mode's closing residual: the FP64-anchored update (one FP64
 product per solve) against the recomputed FP64 residual, and every recorded
 step re-assembled and re-checked in FP64 (TimeStepper.verify)."""
-import numpy as np
 import pytest
 import torch
 
