@@ -48,6 +48,8 @@ SIGNATURES = {
     "f3d_periphery_apply": (ctypes.c_int, [
         ctypes.c_int, _i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
         _i64, _i64, _f64, _f64, _f64, _c_p, _c_p, _sz, _c_p]),
+    "f3d_oseen_matrix": (ctypes.c_int, [
+        _i64, _c_p, _i64, _i64, _i64, _c_p, _i64, _i64, _f64, _f64, _f64, _i32, _c_p, _i64, _c_p]),
     "f3d_stresslet_matrix": (ctypes.c_int, [
         _i64, _c_p, _c_p, _i64, _i64, _f64, _f64, _f64, _c_p, _i64, _c_p]),
     "f3d_lu_factor_batched": (ctypes.c_int, [

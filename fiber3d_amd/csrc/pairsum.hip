@@ -774,7 +774,59 @@ stresslet_matrix_kernel(int n, const double* __restrict__ pos, const double* __r
     for (int b = 0; b < 3; ++b) row[b] = c * d[a] * d[b];
   }
 }
+// Dense Oseen tensor between nt targets and ns sources: block (i, j) =
+// (I / r + d d^T / r^3) / (8 pi eta), d = t_i - s_j, with r replaced by
+// sqrt(r^2 + reg^2) where r <= eps (the dense builders regularise at r <= eps,
+// the sums at r < eps).  layout 0 (AoS, 'r'): entry (3 i + a, 3 j + b);
+// layout 1 ('xyz'): entry (a nt + i, b ns + j).  One thread per (i, j).
+__global__ void __launch_bounds__(256)
+oseen_matrix_kernel(int ns, const double* __restrict__ src, long ssp, long ssc, int nt,
+                    const double* __restrict__ trg, long tsp, long tsc, double c8, double reg2,
+                    double eps, int layout, double* __restrict__ M, long ld) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= ns) return;
+  const long bi = (long)i * tsp, bj = (long)j * ssp;
+  const double d[3] = {trg[bi] - src[bj], trg[bi + tsc] - src[bj + ssc],
+                       trg[bi + 2 * tsc] - src[bj + 2 * ssc]};
+  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double r = sqrt(r2);
+  if (!(r > eps)) {
+    r2 += reg2;
+    r = sqrt(r2);
+  }
+  const double fr = c8 / r, gr = c8 / (r2 * r);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const double v = (a == b ? fr : 0.0) + gr * d[a] * d[b];
+      if (layout == 0)
+        M[(3L * i + a) * ld + 3L * j + b] = v;
+      else
+        M[((long)a * nt + i) * ld + (long)b * ns + j] = v;
+    }
+  }
+}
 }  // namespace f3d
+
+extern "C" int f3d_oseen_matrix(int64_t ns, const double* src, int64_t s_sp, int64_t s_sc,
+                                int64_t nt, const double* trg, int64_t t_sp, int64_t t_sc,
+                                double eta, double reg, double eps, int32_t layout, double* M,
+                                int64_t ld, void* stream) {
+  F3D_REQUIRE(ns >= 0 && ns < (1ll << 30) && nt >= 0 && nt < (1ll << 30), F3D_EINVAL,
+              "oseen_matrix: bad sizes ns=%lld nt=%lld", (long long)ns, (long long)nt);
+  F3D_REQUIRE(layout == 0 || layout == 1, F3D_EINVAL, "oseen_matrix: layout is 0 or 1");
+  F3D_REQUIRE(ld >= 3 * ns, F3D_EINVAL, "oseen_matrix: ld < 3 ns");
+  F3D_REQUIRE(eta > 0.0, F3D_EINVAL, "oseen_matrix: eta must be positive");
+  if (ns == 0 || nt == 0) return F3D_OK;
+  F3D_REQUIRE(src && trg && M, F3D_EINVAL, "oseen_matrix: NULL argument");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  F3D_LAUNCH(oseen_matrix_kernel, dim3((unsigned)ceil_div(ns, 256), (unsigned)nt), dim3(256), 0,
+             st, (int)ns, src, (long)s_sp, (long)s_sc, (int)nt, trg, (long)t_sp, (long)t_sc,
+             1.0 / (8.0 * kPi * eta), reg * reg, eps, (int)layout, M, (long)ld);
+  return F3D_OK;
+}
 
 extern "C" int f3d_stresslet_matrix(int64_t n, const double* pos, const double* nrm, int64_t sp,
                                     int64_t sc, double eta, double reg, double eps, double* M,

@@ -159,3 +159,22 @@ class DenseGemv:
         L.check(rc, "f3d_dense_gemv_batched")
         return y
 
+
+
+def build_block_diagonal_lu_preconditioner_fiber(fibers, A_fibers_blocks):
+    """Drop-in for tstep_utils.build_block_diagonal_lu_preconditioner_fiber
+    (tstep/tstep_utils.py:1405-1419): (LU_all, P_all), per fiber the pair that
+    scipy.linalg.lu_factor(A, check_finite=False) returns (packed LU, 0-based
+    row interchanges), from one batched FP64 factorisation on the GPU with
+    LAPACK's partial pivoting.  The reference's lu_solve((LU, P), x) calls
+    apply them unchanged; fiber3d_amd.dense.BatchedLU keeps the factors on the
+    device instead."""
+    mats = [np.asarray(A, dtype=np.float64) for A in A_fibers_blocks]
+    if not mats:
+        return [], []
+    bs = BlockSet([A.shape[0] for A in mats])
+    lu = BatchedLU(bs, bs.pack(mats), precision="f64")
+    LU_all = bs.unpack(lu.lu64)
+    piv = lu.piv.cpu().numpy()
+    P_all = [piv[o:o + n].astype(np.int32) for o, n in zip(bs.x_off_h, bs.sizes)]
+    return LU_all, P_all

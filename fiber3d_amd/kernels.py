@@ -232,3 +232,60 @@ def complementary_kernel_times_density_numba(r_vectors, normal, density):
     n = _to_dev(normal)
     q = _to_dev(density, n.shape[0])
     return (n * (n * q).sum()).cpu().numpy().ravel()
+
+
+def _points(r, fmt, what):
+    """Device (N, 3) view parameters of an AoS ('r') or component-block
+    ('xyz') point array: (tensor, point stride, component stride, N)."""
+    a = np.ascontiguousarray(np.asarray(r, dtype=np.float64).ravel())
+    if a.size % 3:
+        raise ValueError(f"{what}: size {a.size} is not a multiple of 3")
+    n = a.size // 3
+    if fmt == "r":
+        return torch.from_numpy(a).to(device()), 3, 1, n
+    if fmt == "xyz":
+        return torch.from_numpy(a).to(device()), 1, n, n
+    raise ValueError("input_format is 'r' or 'xyz'")
+
+
+def oseen_tensor_source_target(r_source, r_target, eta=1.0, reg=5e-3, epsilon_distance=1e-5,
+                               input_format="r"):
+    """Drop-in for kernels.oseen_tensor_source_target (kernels/kernels.py:119-211):
+    the dense (3 Nt, 3 Ns) Oseen tensor, built on the GPU by f3d_oseen_matrix."""
+    s, ssp, ssc, ns = _points(r_source, input_format, "r_source")
+    t, tsp, tsc, nt = _points(r_target, input_format, "r_target")
+    G = torch.empty((3 * nt, 3 * ns), dtype=torch.float64, device=device())
+    rc = L.load().f3d_oseen_matrix(ns, L.ptr(s), ssp, ssc, nt, L.ptr(t), tsp, tsc, float(eta),
+                                   float(reg), float(epsilon_distance),
+                                   0 if input_format == "r" else 1, L.ptr(G), 3 * ns, _stream())
+    L.check(rc, "f3d_oseen_matrix")
+    return G.cpu().numpy()
+
+
+def oseen_tensor(r_vectors, eta=1.0, reg=5e-3, epsilon_distance=1e-5, input_format="r"):
+    """Drop-in for kernels.oseen_tensor (kernels/kernels.py:31-116): the points
+    as both sources and targets (diagonal blocks regularised, not zero)."""
+    return oseen_tensor_source_target(r_vectors, r_vectors, eta, reg, epsilon_distance,
+                                      input_format)
+
+
+def stresslet_kernel_times_normal_numba(r_vectors, normal, eta=1.0, reg=5e-3,
+                                        epsilon_distance=1e-5):
+    """Drop-in for kernels.stresslet_kernel_times_normal_numba
+    (kernels/kernels.py:1063-1120): the dense (3N, 3N) stresslet contracted with
+    the normals, built on the GPU by f3d_stresslet_matrix."""
+    x = _to_dev(r_vectors)
+    n = _to_dev(normal, x.shape[0])
+    N = x.shape[0]
+    M = torch.empty((3 * N, 3 * N), dtype=torch.float64, device=device())
+    rc = L.load().f3d_stresslet_matrix(N, L.ptr(x), L.ptr(n), 3, 1, float(eta), float(reg),
+                                       float(epsilon_distance), L.ptr(M), 3 * N, _stream())
+    L.check(rc, "f3d_stresslet_matrix")
+    return M.cpu().numpy()
+
+
+def complementary_kernel(r_vectors, normal):
+    """Drop-in for kernels.complementary_kernel (kernels/kernels.py:1286-1295):
+    the rank-one (3N, 3N) matrix n n^T, formed on the GPU."""
+    n = _to_dev(normal).reshape(-1)
+    return torch.outer(n, n).cpu().numpy()

@@ -142,6 +142,21 @@ int f3d_periphery_apply(int precision, int64_t n, const double* pos, const doubl
                         double* out, void* ws, size_t ws_bytes, void* stream);
 
 /*
+ * Dense Oseen tensor, row-major with leading dimension ld: block (i, j) =
+ * (I/r + d d^T/r^3)/(8 pi eta) between target i and source j, d = t_i - s_j,
+ * r regularised to sqrt(r^2 + reg^2) where r <= eps.  Point k of the sources
+ * is (src[k s_sp], src[k s_sp + s_sc], src[k s_sp + 2 s_sc]) (AoS: 3, 1;
+ * 'xyz' blocks: 1, ns), likewise the targets.  layout 0: entry (3i+a, 3j+b)
+ * ('r' format); layout 1: entry (a nt + i, b ns + j) ('xyz' format).
+ * Replaces kernels.oseen_tensor and kernels.oseen_tensor_source_target
+ * (python/kernels/kernels.py:31-116, :119-211), used for the per-fiber self
+ * blocks (python/tstep/tstep_utils.py:717-722, :794-800).
+ */
+int f3d_oseen_matrix(int64_t ns, const double* src, int64_t s_sp, int64_t s_sc, int64_t nt,
+                     const double* trg, int64_t t_sp, int64_t t_sc, double eta, double reg,
+                     double eps, int32_t layout, double* M, int64_t ld, void* stream);
+
+/*
  * Dense 3n x 3n stresslet-times-normal matrix, row-major with leading
  * dimension ld: block (i, j != i) = (-3/(4 pi eta)) (d . n_j)/r^5 d d^T with
  * d = x_i - x_j (regularised for r < eps), zero diagonal blocks.  Replaces

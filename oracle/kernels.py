@@ -151,19 +151,36 @@ def complementary_kernel_times_density(r_vectors, normal, density):
     return (n * np.sum(n * q)).ravel()
 
 
-def oseen_tensor(r_vectors, eta=1.0, reg=5e-3, epsilon_distance=1e-5):
-    """Dense 3N x 3N Oseen tensor of N points, AoS ordering ('r' format).
+def _xyz_points(r, fmt):
+    """(N, 3) points from AoS ('r') or concatenated x, y, z blocks ('xyz')."""
+    r = np.asarray(r, dtype=np.float64).ravel()
+    if fmt == "r":
+        return r.reshape(-1, 3)
+    return np.ascontiguousarray(r.reshape(3, -1).T)
 
-    Follows kernels.oseen_tensor (kernels/kernels.py:31-116): entries with
-    r <= eps (note <=, unlike the numba sums) use sqrt(r^2+reg^2); the
-    diagonal block is I/(8 pi eta reg).
-    """
-    x = _pts(r_vectors)
-    N = x.shape[0]
-    d = x[None, :, :] - x[:, None, :]
+
+def oseen_tensor_source_target(r_source, r_target, eta=1.0, reg=5e-3, epsilon_distance=1e-5,
+                               input_format="r"):
+    """Dense 3Nt x 3Ns Oseen tensor (kernels/kernels.py:119-211): block (i, j)
+    = (I/r + d d^T/r^3)/(8 pi eta), d = t_i - s_j; entries with r <= eps
+    (note <=, unlike the numba sums) use sqrt(r^2 + reg^2).  'r' orders rows
+    and columns point by point, 'xyz' component block by component block."""
+    s, t = _xyz_points(r_source, input_format), _xyz_points(r_target, input_format)
+    Ns, Nt = s.shape[0], t.shape[0]
+    d = t[:, None, :] - s[None, :, :]
     r = np.sqrt(np.einsum("ijk,ijk->ij", d, d))
     r = np.where(r > epsilon_distance, r, np.sqrt(r * r + reg * reg))
     fr = 1.0 / (8.0 * np.pi * eta * r)
     gr = 1.0 / (8.0 * np.pi * eta * r**3)
     G = fr[:, :, None, None] * np.eye(3)[None, None] + gr[:, :, None, None] * d[:, :, :, None] * d[:, :, None, :]
-    return G.transpose(0, 2, 1, 3).reshape(3 * N, 3 * N)
+    if input_format == "r":
+        return G.transpose(0, 2, 1, 3).reshape(3 * Nt, 3 * Ns)
+    return G.transpose(2, 0, 3, 1).reshape(3 * Nt, 3 * Ns)
+
+
+def oseen_tensor(r_vectors, eta=1.0, reg=5e-3, epsilon_distance=1e-5, input_format="r"):
+    """Dense 3N x 3N Oseen tensor of N points (kernels/kernels.py:31-116): the
+    source-target tensor with the points as both; the diagonal block is
+    I/(8 pi eta reg)."""
+    return oseen_tensor_source_target(r_vectors, r_vectors, eta, reg, epsilon_distance,
+                                      input_format)

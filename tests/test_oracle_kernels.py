@@ -128,6 +128,23 @@ def test_oseen_tensor_dense_matches_sum():
     np.testing.assert_allclose(np.diag(G)[:3], 1.0 / (8 * np.pi * 5e-3), rtol=1e-14)
 
 
+def test_oseen_tensor_source_target_and_xyz_format():
+    """Source-target tensor times a density equals the pair sum
+    (kernels.py:119-211 against :794-846); the 'xyz' format is the 'r' one with
+    rows and columns regrouped by component."""
+    rng = np.random.default_rng(17)
+    s, t = rng.normal(size=(30, 3)), rng.normal(size=(20, 3))
+    f = rng.normal(size=(30, 3))
+    G = K.oseen_tensor_source_target(s, t)
+    assert G.shape == (60, 90)
+    assert rel2(G @ f.ravel(), K.oseen_kernel_source_target(s, t, f)) < 1e-13
+    Gx = K.oseen_tensor_source_target(s.T.ravel(), t.T.ravel(), input_format="xyz")
+    rows = np.arange(60).reshape(20, 3).T.ravel()   # 'xyz' row a*Nt+i is 'r' row 3i+a
+    cols = np.arange(90).reshape(30, 3).T.ravel()
+    assert np.array_equal(Gx, G[np.ix_(rows, cols)])
+    assert rel2(Gx @ f.T.ravel(), (G @ f.ravel()).reshape(20, 3).T.ravel()) < 1e-13
+
+
 def test_shell_dense_equals_matrix_free_and_invariant():
     R = 3.0
     x, nout, w = _sphere(300, R)
