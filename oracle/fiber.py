@@ -232,10 +232,16 @@ def compute_RHS(fm, x, length, dt, c0, c1, penalty=500.0, force=None, flow=None,
 
 
 def apply_BC_rectangular(fm, x, length, E, A, RHS, end_force=None,
-                         start_force=None, start_torque=None, end_torque=None):
+                         start_force=None, start_torque=None, end_torque=None,
+                         clamped=False, dt=None, c0=None, flow=None, force=None):
     """Square system [P_cheb A; B], [P_cheb RHS; B_RHS] for force/torque-free
     ends with optional prescribed end forces/torques (fiber.py:810-961, the
-    'force' and 'torque' branches)."""
+    'force' and 'torque' branches).  clamped=True replaces the minus-end rows
+    by the 'velocity' and 'angular_velocity' branches (fiber.py:855-877,
+    :887-898) with zero prescribed velocities, as tstep sets them for a fiber
+    attached to a body (tstep.py:878-881): x(0)/dt and x_s(0)/dt rows, and
+    the tension row 6 E c0 x_ss(0) . D3 X + 2 c0 D1 T whose right-hand side
+    carries -(flow(0) + 2 c0 force(0)) . x_s(0)."""
     N = fm.num_points
     D1, D2, D3, _ = fm.Ds(length)
     xs, xss, _, _ = derivatives(fm, x, length)
@@ -263,6 +269,20 @@ def apply_BC_rectangular(fm, x, length, E, A, RHS, end_force=None,
     Bv[7:10] = F1
     Bv[10] = np.dot(F1, xs[-1])
     Bv[11:14] = L1
+    if clamped:
+        B[0:7] = 0.0
+        for a in range(3):
+            B[a, a * N] = 1.0 / dt
+            B[3, a * N:(a + 1) * N] = 6.0 * E * c0 * xss[0, a] * D3[0]
+            B[4 + a, a * N:(a + 1) * N] = D1[0] / dt
+        B[3, 3 * N:] = 2.0 * c0 * D1[0]
+        Bv[0:3] = x[0] / dt
+        Bv[3] = 0.0
+        if flow is not None:
+            Bv[3] -= np.dot(np.asarray(flow, float)[0], xs[0])
+        if force is not None:
+            Bv[3] -= 2.0 * c0 * np.dot(np.asarray(force, float)[0], xs[0])
+        Bv[4:7] = xs[0] / dt
     A_sq = np.vstack((fm.P_cheb @ A, B))
     b = np.concatenate((fm.P_cheb @ RHS, Bv))
     return A_sq, b

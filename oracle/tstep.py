@@ -38,10 +38,13 @@ from . import periphery as P
 class HydroSystem:
     def __init__(self, fibers, shell=None, eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
-                 lengths=None, bodies=None, body_force=None):
+                 lengths=None, bodies=None, body_force=None, attached=None):
         """fibers: list of (N_k,3) arrays; shell: (x, normals, weights) or None;
         bodies: list of (centre (3,), nodes (Nb,3), outward normals, weights);
-        body_force: optional (nbody, 6) external force and torque per body."""
+        body_force: optional (nbody, 6) external force and torque per body;
+        attached: optional list of (fiber k, body j, link (3,)): fiber k's
+        minus end is clamped to body j at centre_j + link (tstep.py:878-881,
+        tstep_utils.build_link_matrix :948-1091)."""
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = OF.constants(epsilon, eta)
         self.fibers = [np.asarray(x, float) for x in fibers]
@@ -61,6 +64,9 @@ class HydroSystem:
         self.fib0 = 3 * self.nper + 3 * int(self.boffs[-1]) + 6 * self.nbody
         self.size = self.fib0 + 4 * self.npts
         self.trg_fib = np.concatenate(self.fibers) if self.fibers else np.zeros((0, 3))
+        self.attached = [] if attached is None else [(int(k), int(j), np.asarray(l, float))
+                                                     for k, j, l in attached]
+        self.clamped = {k for k, _, _ in self.attached}
         self._setup(end_force)
 
     # -------------------------------------------------------------------
@@ -130,7 +136,9 @@ class HydroSystem:
             A = OF.form_linear_operator(fm, x, L, self.E, self.dt, self.c0, self.c1, self.penalty)
             R = OF.compute_RHS(fm, x, L, self.dt, self.c0, self.c1, self.penalty, force=f_k,
                                flow=v_k)
-            A_sq, b = OF.apply_BC_rectangular(fm, x, L, self.E, A, R, end_force=f_k[-1])
+            A_sq, b = OF.apply_BC_rectangular(fm, x, L, self.E, A, R, end_force=f_k[-1],
+                                              clamped=k in self.clamped, dt=self.dt,
+                                              c0=self.c0, flow=v_k, force=f_k)
             self.A_sq.append(A_sq)
             o = self.fib0 + 4 * self.offs[k]
             rhs[o:o + 4 * N] = b
@@ -213,12 +221,40 @@ class HydroSystem:
             vel.append(x[o + 3 * n:o + 3 * n + 6])
         return lam, vel
 
+    def _link_forces(self, XT):
+        """Force and torque of every attached fiber on its body (the body rows
+        of tstep_utils.build_link_matrix, tstep_utils.py:995-1036):
+        F = -(E X_sss(0) - T(0) x_s(0)), L = link x F + E x_s(0) x X_ss(0),
+        summed per body -> (nbody, 6)."""
+        out = np.zeros((self.nbody, 6))
+        for k, j, link in self.attached:
+            N = self.sizes[k]
+            o = 4 * self.offs[k]
+            X = XT[o:o + 3 * N].reshape(3, N)
+            T0 = XT[o + 3 * N]
+            _, D2, D3, _ = self.fm[N].Ds(self.lengths[k])
+            xs0 = self.derivs[k][0][0]
+            F = -self.E * (X @ D3[0]) + T0 * xs0
+            out[j, :3] += F
+            out[j, 3:] += np.cross(link, F) + self.E * np.cross(xs0, X @ D2[0])
+        return out
+
     def matvec(self, x):
         """A_body_fiber_hyd (tstep.py:1003-1178) for fibers, free bodies and shell."""
         q, XT = self.split(x)
         lam, vel = self.body_parts(x)
         eta = self.eta
         res = np.zeros_like(x)
+        centres = np.array([b[0] for b in self.bodies]).reshape(-1, 3)
+        link_ft = self._link_forces(XT) if self.attached else None
+
+        def link_flow(trg):
+            """Stokeslet + rotlet of the fibers' forces and torques on the
+            bodies, at the body centres (tstep.py:1052-1064)."""
+            if link_ft is None or trg.shape[0] == 0:
+                return 0.0
+            return (K.oseen_kernel_source_target(centres, trg, link_ft[:, :3], eta)
+                    + K.rotlet_kernel_source_target(centres, trg, link_ft[:, 3:], eta))
         f = np.zeros((self.npts, 3))
         for k in range(len(self.fibers)):
             N = self.sizes[k]
@@ -233,9 +269,12 @@ class HydroSystem:
             if self.npts:
                 v_fib = v_fib + K.stresslet_kernel_source_target(
                     self.bnodes, self.trg_fib, self.bnrm, lam_all, eta).reshape(-1, 3)
+                v_fib = v_fib + np.reshape(link_flow(self.trg_fib), (-1, 3)) \
+                    if link_ft is not None else v_fib
             vb = K.stresslet_kernel_source_target(self.bnodes, self.bnodes, self.bnrm, lam_all, eta)
             if self.npts:
                 vb = vb + K.oseen_kernel_source_target(self.trg_fib, self.bnodes, fw, eta)
+            vb = vb + link_flow(self.bnodes)
         if self.shell is not None:
             xs_, ns_, ws_ = self.shell
             sh = self.M_shell @ q.ravel()
@@ -246,6 +285,7 @@ class HydroSystem:
             if self.nbody:
                 sh = sh + K.stresslet_kernel_source_target(self.bnodes, xs_, self.bnrm, lam_all, eta)
                 vb = vb + K.stresslet_kernel_source_target(xs_, self.bnodes, ns_, q, eta)
+                sh = sh + link_flow(xs_)
             res[:3 * self.nper] = sh
         for k in range(self.nbody):
             o = self._body_off(k)
@@ -264,6 +304,18 @@ class HydroSystem:
             vk = v_fib[self.offs[k]:self.offs[k + 1]].T.ravel()
             res[self.fib0 + o:self.fib0 + o + 4 * N] = \
                 self.A_sq[k] @ XT[o:o + 4 * N] - self.Q[k] @ vk
+        # clamped minus ends: the fiber rows of the link matrix
+        # (tstep_utils.py:1039-1088) and the flow term of the tension condition
+        # (tstep.py:1162-1166)
+        for k, j, link in self.attached:
+            N = self.sizes[k]
+            r0 = self.fib0 + 4 * self.offs[k] + 4 * N - 14
+            U, W = vel[j][:3], vel[j][3:]
+            xs0 = self.derivs[k][0][0]
+            vU = U + np.cross(W, link)
+            res[r0:r0 + 3] -= vU
+            res[r0 + 3] += -np.dot(vU, xs0) + np.dot(v_fib[self.offs[k]], xs0)
+            res[r0 + 4:r0 + 7] -= np.cross(W, link / np.linalg.norm(link))
         return res
 
     def precond(self, x):

@@ -8,7 +8,8 @@ fibers, the inputs and the reference outputs of:
   fiber_matrices.compute_matrices / get_matrices  (tstep/fiber_matrices.py:38-153)
   fiber.form_linear_operator                      (fiber/fiber.py:322-447)
   fiber.compute_RHS                               (fiber/fiber.py:473-582)
-  fiber.apply_BC_rectangular                      (fiber/fiber.py:810-961)
+  fiber.apply_BC_rectangular                      (fiber/fiber.py:810-961),
+                                                  free and clamped minus end
   fiber.force_operator                            (fiber/fiber.py:1399-1431)
 Only arrays are stored (no pickles); the reference itself is not copied.
 """
@@ -46,6 +47,10 @@ def main():
         RHS = fib.compute_RHS(fib_mat=fm, force_external=force, flow=flow)
         fib.set_BC(BC_end_vec_0=force[-1])
         A_sq, b = fib.apply_BC_rectangular(A, RHS, fm, flow, force)
+        # minus end clamped (a fiber attached to a body, tstep.py:878-881)
+        fib.set_BC(BC_start_0="velocity", BC_start_1="angular_velocity",
+                   BC_end_vec_0=force[-1])
+        A_cl, b_cl = fib.apply_BC_rectangular(A, RHS, fm, flow, force)
         F = fib.force_operator(fm)
         w, _, _, _ = fm.get_matrices(L, N, "weights_all")
         _, _, P_cheb, _ = fm.get_matrices(L, N, "PX_PT_Pcheb")
@@ -55,7 +60,8 @@ def main():
                      p + "x": x, p + "force": force, p + "flow": flow,
                      p + "D1": D1, p + "D2": D2, p + "D3": D3, p + "D4": D4,
                      p + "weights": w, p + "P_cheb": P_cheb, p + "A": A, p + "RHS": RHS,
-                     p + "A_sq": A_sq, p + "b": b, p + "F": F})
+                     p + "A_sq": A_sq, p + "b": b, p + "F": F,
+                     p + "A_sq_clamped": A_cl, p + "b_clamped": b_cl})
     np.savez_compressed(OUT, **data)
     print(f"wrote {OUT} ({os.path.getsize(OUT) / 1e3:.0f} kB)")
 

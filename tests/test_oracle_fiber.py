@@ -43,6 +43,15 @@ def test_fiber_assembly_matches_reference_golden(gold, ci):
     assert rel2(A_sq, g["A_sq"]) < 1e-13
     assert rel2(b, g["b"]) < 1e-13
     assert rel2(OF.force_operator(fm, x, L, E), g["F"]) < 1e-13
+    # minus end clamped to a body: 'velocity' + 'angular_velocity' rows
+    A_cl, b_cl = OF.apply_BC_rectangular(fm, x, L, E, A, R, end_force=g["force"][-1],
+                                         clamped=True, dt=dt, c0=c0, flow=g["flow"],
+                                         force=g["force"])
+    assert rel2(A_cl, g["A_sq_clamped"]) < 1e-13
+    assert rel2(b_cl, g["b_clamped"]) < 1e-13
+    # only the seven minus-end rows differ from the free fiber
+    assert np.array_equal(A_cl[:4 * N - 14], A_sq[:4 * N - 14])
+    assert np.array_equal(A_cl[4 * N - 7:], A_sq[4 * N - 7:])
 
 
 @pytest.mark.parametrize("N", [16, 32, 64])

@@ -80,3 +80,73 @@ def test_body_preconditioner_inverts_its_block():
     x = np.random.default_rng(4).normal(size=S.size)
     # the body block alone is the whole operator here
     assert rel2(S.precond(S.matvec(x)), x) < 1e-9
+
+
+def _clamped_assembly(E, force, torque=(0.0, 0.0, 0.0), nb=400, N=16, L=1.0, dt=1e-3):
+    """A sphere (radius 0.5, centre 0) with one fiber clamped radially at its
+    top, pointing along +z, no periphery, an external force and torque on the
+    sphere; one implicit step solved to 1e-10."""
+    a = 0.5
+    body = configs.make_sphere_body([0.0, 0.0, 0.0], a, nb)
+    s = np.linspace(0.0, L, N)
+    fiber = np.stack([np.zeros(N), np.zeros(N), a + s], axis=1)
+    S = OT.HydroSystem([fiber], None, eta=1.0, dt=dt, E=E, end_force=0.0, bodies=[body],
+                       body_force=[np.concatenate([force, torque])],
+                       attached=[(0, 0, np.array([0.0, 0.0, a]))])
+    x, info, _ = OT.gmres(S.matvec, S.rhs, S.precond, tol=1e-10)
+    assert info == 0
+    return S, x, fiber
+
+
+def test_clamped_fiber_link_balances_the_fiber_forces():
+    """The link's force and torque on the body (tstep_utils.py:995-1036) are
+    the fiber's end force and moment: with a free plus end they balance the
+    fiber's whole force density, sum_i w_i f_i = -F_link and
+    sum_i w_i X_i x f_i = -L_link, up to the quadrature error (a sign or a
+    misplaced term in the link breaks this at O(1))."""
+    S, x, fiber = _clamped_assembly(100.0, np.array([0.3, -0.5, 0.2]),
+                                    torque=(0.2, 0.4, -0.7), dt=0.05)
+    _, XT = S.split(x)
+    ft = S._link_forces(XT)[0]
+    N = fiber.shape[0]
+    f = (S.F_blocks[0] @ XT[:4 * N]).reshape(3, N).T
+    X = XT[:3 * N].reshape(3, N).T
+    w = S.weights[0][:, None]
+    assert np.linalg.norm((w * f).sum(0) + ft[:3]) < 2e-2 * np.linalg.norm(ft[:3])
+    assert np.linalg.norm((w * np.cross(X, f)).sum(0) + ft[3:]) < 2e-2 * np.linalg.norm(ft[3:])
+
+
+def test_clamped_fiber_moves_with_its_body():
+    """A stiff fiber clamped to a sphere (relaxation time well below dt)
+    moves with it almost rigidly; its drag slows the sphere below Stokes'
+    law and, acting above the centre, tilts it (Omega_y < 0 for a force
+    along +x)."""
+    F = np.array([1.0, 0.0, 0.0])
+    S, x, fiber = _clamped_assembly(100.0, F, dt=0.05)
+    _, vel = S.body_parts(x)
+    U, W = vel[0][:3], vel[0][3:]
+    free = F / (6 * np.pi * 0.5)
+    assert U[0] < 0.99 * free[0]
+    assert abs(U[1]) < 1e-3 * U[0] and abs(W[0]) < 1e-3 * U[0] and W[1] < 0.0
+    N = fiber.shape[0]
+    Xn = x[S.fib0:S.fib0 + 3 * N].reshape(3, N).T
+    v_fib = (Xn - fiber) / S.dt
+    v_rigid = U + np.cross(W, fiber)
+    assert np.linalg.norm(v_fib - v_rigid) < 2e-2 * np.linalg.norm(v_rigid)
+    # the clamped end itself moves exactly with the sphere's surface point
+    assert np.allclose(v_fib[0], U + np.cross(W, [0.0, 0.0, 0.5]), rtol=0, atol=1e-8)
+
+
+def test_clamped_fiber_torque_turns_the_assembly():
+    """A torque about x on the sphere turns sphere and fiber together: the
+    fiber's free end moves along -y (Omega_x > 0 carries +z towards -y)."""
+    S, x, fiber = _clamped_assembly(100.0, np.zeros(3), torque=(1.0, 0.0, 0.0), dt=0.05)
+    _, vel = S.body_parts(x)
+    U, W = vel[0][:3], vel[0][3:]
+    assert W[0] > 0.0 and abs(W[1]) < 1e-3 * W[0]
+    N = fiber.shape[0]
+    Xn = x[S.fib0:S.fib0 + 3 * N].reshape(3, N).T
+    tip_v = (Xn[-1] - fiber[-1]) / S.dt
+    assert tip_v[1] < 0.0
+    # the tip, three radii out, lags the rigid rotation a little (bending)
+    assert np.linalg.norm(tip_v - U - np.cross(W, fiber[-1])) < 0.15 * np.linalg.norm(tip_v)
