@@ -350,7 +350,7 @@ class FiberSystem:
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
                  lengths=None, pair_precision="f64", lu_precision="f64", lu_refine=6,
                  lu_tol=1e-13, reg=5e-3, eps=1e-5, bodies=None, body_force=None,
-                 assembly="hip"):
+                 assembly="hip", attached=None):
         self.dev = device()
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = _c0c1(epsilon, eta)
@@ -428,6 +428,7 @@ class FiberSystem:
                 "PX": P[:N - 4, :N].contiguous(), "PT": P[3 * (N - 4):, 3 * N:].contiguous(),
                 "length": self.lengths[bk["ids_t"]].contiguous(),
             }
+        self._setup_attached(attached)
         # work buffers
         z = lambda n: torch.zeros(n, dtype=torch.float64, device=self.dev)
         self.f_soa = z(3 * self.npts)
@@ -436,6 +437,122 @@ class FiberSystem:
         self.prepare()
 
     # ------------------------------------------------------------------
+    def _setup_attached(self, attached):
+        """Fibers whose minus end is clamped to a body (tstep.py:878-881):
+        attached = [(fiber k, body j, link (3,))], the link being the
+        nucleation site relative to the body centre in the body's pose at
+        construction.  Grouped by node count for the batched row patches."""
+        self.att = {}
+        if not attached:
+            return
+        if self.bodies is None:
+            raise ValueError("attached fibers need bodies")
+        ak = np.array([int(a[0]) for a in attached])
+        aj = np.array([int(a[1]) for a in attached])
+        lref = np.array([np.asarray(a[2], float) for a in attached]).reshape(-1, 3)
+        if len(set(ak.tolist())) != ak.size or ak.min() < 0 or ak.max() >= self.nfib:
+            raise ValueError("attached: fiber indices must be distinct and in range")
+        if aj.min() < 0 or aj.max() >= self.nbody:
+            raise ValueError("attached: body index out of range")
+        if np.any(np.linalg.norm(lref, axis=1) == 0.0):
+            raise ValueError("attached: a link must not sit at the body centre")
+        ti = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int64)).to(self.dev)
+        for N, bk in self.buckets.items():
+            sel = np.flatnonzero(np.isin(ak, bk["ids"]))
+            if sel.size == 0:
+                continue
+            fib = ak[sel]
+            self.att[N] = {
+                "fib": fib, "pib": ti(np.searchsorted(bk["ids"], fib)), "body": ti(aj[sel]),
+                "lref": torch.from_numpy(lref[sel]).to(self.dev),
+                "node0": ti(self.offs[fib]), "xo": ti(4 * self.offs[fib]),
+                "a_off": ti(self.bs.a_off_h[fib]),
+                "row0": ti(self.fib0 + 4 * self.offs[fib] + 4 * N - 14),
+            }
+
+    def _links(self, g):
+        """Current link vectors (rotated with their bodies) and directions."""
+        link = torch.einsum("nij,nj->ni", self.bodies.rot[g["body"]], g["lref"])
+        return link, link / torch.linalg.norm(link, dim=1, keepdim=True)
+
+    def _apply_clamped(self):
+        """Replace the minus-end boundary rows of every attached fiber by the
+        'velocity' and 'angular_velocity' rows (fiber.py:855-877, :887-898):
+        x(0)/dt, x_s(0)/dt, and the tension row 6 E c0 x_ss(0).D3 X +
+        2 c0 D1 T with right-hand side -(v(0) + 2 c0 f(0)).x_s(0)."""
+        dt, E, c0 = self.dt, self.E, self.c0
+        for N, g in self.att.items():
+            node, X, xs, xss, D1, P, B = self._ctx[N]
+            bk = self.buckets[N]
+            sc = 2.0 / self.lengths[bk["ids_t"][g["pib"]]]
+            d1 = bk["D0"][0][0][None, :] * sc[:, None]
+            d2 = bk["D0"][1][0][None, :] * sc[:, None] ** 2
+            d3 = bk["D0"][2][0][None, :] * sc[:, None] ** 3
+            xs0, xss0 = xs[g["pib"], 0], xss[g["pib"], 0]
+            na, n = len(g["fib"]), 4 * N
+            R = torch.zeros((na, 7, n), dtype=torch.float64, device=self.dev)
+            for a in range(3):
+                R[:, a, a * N] = 1.0 / dt
+                R[:, 3, a * N:(a + 1) * N] = 6.0 * E * c0 * xss0[:, a, None] * d3
+                R[:, 4 + a, a * N:(a + 1) * N] = d1 / dt
+            R[:, 3, 3 * N:] = 2.0 * c0 * d1
+            # column-major blocks: entry (row, col) at a_off + col * n + row
+            rows = torch.arange(n - 14, n - 7, device=self.dev)
+            cols = torch.arange(n, device=self.dev)
+            idx = g["a_off"][:, None, None] + cols[None, None, :] * n + rows[None, :, None]
+            self.A_buf[idx.reshape(-1)] = R.reshape(-1)
+            r0 = g["row0"]
+            X0 = self.X[g["node0"]]
+            v0, f0 = self.v_explicit[g["node0"]], self.force[g["node0"]]
+            for a in range(3):
+                self.rhs[r0 + a] = X0[:, a] / dt
+                self.rhs[r0 + 4 + a] = xs0[:, a] / dt
+            self.rhs[r0 + 3] = -((v0 + 2.0 * c0 * f0) * xs0).sum(1)
+            g["xs0"], g["d2"], g["d3"] = xs0.clone(), d2, d3
+
+    def _link_forces(self, XT):
+        """Force and torque of the attached fibers on their bodies (the body
+        rows of tstep_utils.build_link_matrix, tstep_utils.py:995-1036):
+        F = -E X_sss(0) + T(0) x_s(0), L = link x F + E x_s(0) x X_ss(0),
+        summed per body -> (nbody, 6)."""
+        ft = torch.zeros((self.nbody, 6), dtype=torch.float64, device=self.dev)
+        for N, g in self.att.items():
+            blk = XT[g["xo"][:, None] + torch.arange(4 * N, device=self.dev)[None, :]]
+            Xc = blk[:, :3 * N].view(-1, 3, N)
+            T0 = blk[:, 3 * N]
+            F = -self.E * (Xc * g["d3"][:, None, :]).sum(-1) + T0[:, None] * g["xs0"]
+            link, _ = self._links(g)
+            tau = torch.cross(link, F, dim=1) \
+                + self.E * torch.cross(g["xs0"], (Xc * g["d2"][:, None, :]).sum(-1), dim=1)
+            ft.index_add_(0, g["body"], torch.cat([F, tau], dim=1))
+        return ft
+
+    def _link_rows(self, x, out, vsoa):
+        """The fiber rows of the link matrix (tstep_utils.py:1039-1088):
+        -(U + Omega x link) in the position rows, -(U + Omega x link).x_s(0)
+        plus the flow term v(0).x_s(0) (tstep.py:1162-1166) in the tension
+        row, -Omega x link_dir in the orientation rows."""
+        vel = self.bodies._vel(x, 3 * self.nper)
+        for N, g in self.att.items():
+            U, W = vel[g["body"], :3], vel[g["body"], 3:]
+            link, ldir = self._links(g)
+            vU = U + torch.cross(W, link, dim=1)
+            xs0 = g["xs0"]
+            r0 = g["row0"]
+            add = torch.cat([-vU, (-(vU * xs0) + vsoa[g["node0"]] * xs0).sum(1, keepdim=True),
+                             -torch.cross(W, ldir, dim=1)], dim=1)
+            idx = r0[:, None] + torch.arange(7, device=self.dev)[None, :]
+            out.index_add_(0, idx.reshape(-1), add.reshape(-1))
+
+    def fix_links(self):
+        """Translate every attached fiber so that its minus end sits on its
+        body's (rotated) nucleation site (tstep.py:462-470)."""
+        for N, g in self.att.items():
+            link, _ = self._links(g)
+            shift = self.X[g["node0"]] - (self.bodies.centres[g["body"]] + link)
+            nodes = g["node0"][:, None] + torch.arange(N, device=self.dev)[None, :]
+            self.X.index_add_(0, nodes.reshape(-1), -shift.repeat_interleave(N, dim=0))
+
     def _soa(self, buf):
         """(npts, 3) view of an SoA (3, npts) buffer."""
         return buf.view(3, self.npts).t()
@@ -532,6 +649,8 @@ class FiberSystem:
             self.xs[node.reshape(-1)] = xs.reshape(-1, 3)
             self._ctx[N] = (node, X, xs, xss, D1, P, B)
         self._build_rhs()
+        if self.att:
+            self._apply_clamped()
         if getattr(self, "lu", None) is None:
             self.lu = BatchedLU(self.bs, self.A_buf, precision=self.lu_precision,
                                 refine=self.lu_refine, tol=self.lu_tol)
@@ -624,6 +743,14 @@ class FiberSystem:
         fsoa = self._soa(self.f_soa)
         vsoa = self._soa(self.v_soa)
         pairsum("oseen", self.X, fsoa, self.X, src_w=self.weights, excl=self.excl, out=vsoa, **kw)
+        ft = self._link_forces(XT) if self.att else None
+
+        def link_flow(trg, out):
+            # Stokeslet + rotlet of the attached fibers' forces and torques at
+            # the body centres (tstep.py:1052-1064)
+            pairsum("oseen", B.centres, ft[:, :3].contiguous(), trg, out=out, accumulate=True, **kw)
+            pairsum("rotlet", B.centres, ft[:, 3:].contiguous(), trg, out=out, accumulate=True,
+                    **kw)
         if B is not None:
             lam = B.density_view(x, nper3)
             vb = B.vb
@@ -631,6 +758,9 @@ class FiberSystem:
             pairsum("stresslet", B.pos, lam, self.X, src_nrm=B.nrm, out=vsoa, accumulate=True, **kw)
             pairsum("stresslet", B.pos, lam, B.pos, src_nrm=B.nrm, out=vb, **kw)
             pairsum("oseen", self.X, fsoa, B.pos, src_w=self.weights, out=vb, accumulate=True, **kw)
+            if ft is not None:
+                link_flow(self.X, vsoa)
+                link_flow(B.pos, vb)
         if self.shell is not None:
             q = x[:nper3].view(self.nper, 3)
             pairsum("oseen", self.X, fsoa, self.shell.pos, src_w=self.weights, out=self.v_shell, **kw)
@@ -639,6 +769,8 @@ class FiberSystem:
                         accumulate=True, **kw)
                 pairsum("stresslet", self.shell.pos, q, B.pos, src_nrm=self.shell.nrm, out=vb,
                         accumulate=True, **kw)
+                if ft is not None:
+                    link_flow(self.shell.pos, self.v_shell)
             self.shell.apply(q, add=self.v_shell, out=out[:nper3].view(self.nper, 3),
                              precision=prec)
             pairsum("stresslet", self.shell.pos, q, self.X, src_nrm=self.shell.nrm, out=vsoa,
@@ -649,6 +781,8 @@ class FiberSystem:
         self.gA(self.A_buf, XT, res)
         for c in range(3):
             self.gQ[c](self.Q_buf[c], self.v_soa, res, alpha=-1.0, beta=1.0)
+        if self.att:
+            self._link_rows(x, out, vsoa)
         return out
 
     def precond(self, x, out=None):

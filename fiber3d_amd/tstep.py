@@ -15,6 +15,11 @@ Two modes:
              FP64 operator deciding convergence, FP32 fiber LU with FP64
              refinement, and an optional initial-guess predictor.
 
+Fibers may be clamped to rigid bodies (attached=[(fiber, body, link)],
+tstep.py:878-881 with tstep_utils.build_link_matrix :948-1091): their minus
+end follows the body and its tangent turns with it, and their end force and
+moment act on the body.
+
 Time steps are fixed by default.  adaptive=True follows the reference's
 accept/reject scheme (tstep.py:427-581, check_error :1874-1925): after each
 solve the fibers' inextensibility error max |(|x_s| - 1)| of the new
@@ -78,7 +83,7 @@ class TimeStepper:
                  restart=150, maxiter=200, predictor=None, lengths=None, bodies=None,
                  body_force=None, graph=False, certify="update", record=False, lu_every=None,
                  target=None, adaptive=False, tol_tstep=1e-2, dt_min=1e-5, dt_max=1e-2,
-                 periphery_radius=None):
+                 periphery_radius=None, attached=None):
         if mode not in ("plain", "fast"):
             raise ValueError("mode is 'plain' or 'fast'")
         if certify not in ("update", "fp64"):
@@ -112,7 +117,7 @@ class TimeStepper:
                                # a preconditioner need not be exact: the FP32
                                # factors are applied without refinement (a fixed
                                # linear map), the FP64 residual judges convergence
-                               lu_refine=0)
+                               lu_refine=0, attached=attached)
         self.predictor = predictor
         S = self.sys
         # recorded A P^-1 per operator precision, re-recorded after each assembly
@@ -180,6 +185,8 @@ class TimeStepper:
         if S.bodies is not None:
             # rigid motion with the solved velocities (tstep.py:1398-1406)
             S.bodies.move(S.bodies._vel(sol, 3 * S.nper), dt)
+            # attached fibers' minus ends back onto their sites (tstep.py:462-470)
+            S.fix_links()
         # re-assembly refills the system's buffers in place: the recorded
         # Arnoldi graphs stay valid
         self.nsteps += 1
