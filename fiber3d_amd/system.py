@@ -508,7 +508,11 @@ class FiberSystem:
                 self.rhs[r0 + a] = X0[:, a] / dt
                 self.rhs[r0 + 4 + a] = xs0[:, a] / dt
             self.rhs[r0 + 3] = -((v0 + 2.0 * c0 * f0) * xs0).sum(1)
-            g["xs0"], g["d2"], g["d3"] = xs0.clone(), d2, d3
+            # refilled in place: a recorded Arnoldi graph keeps these addresses
+            if "xs0" not in g:
+                g["xs0"], g["d2"], g["d3"] = xs0.clone(), d2, d3
+            else:
+                g["xs0"].copy_(xs0)
 
     def _link_forces(self, XT):
         """Force and torque of the attached fibers on their bodies (the body

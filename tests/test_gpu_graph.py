@@ -101,3 +101,27 @@ def test_graph_scratch_outlives_recording():
         op(torch.from_numpy(np.random.default_rng(1).normal(size=S.size)).cuda())
     torch.cuda.synchronize()
     assert all(float(t.abs().max()) == 0.0 for t in fresh)
+
+
+def test_graph_of_attached_system_survives_steps():
+    """A recorded A P^-1 of a system with fibers clamped to bodies stays equal
+    to the eager operator after steps re-assemble it (the clamped rows, the
+    tangents and the links are refilled in place)."""
+    print("case graph attached", flush=True)
+    from test_gpu_attached import _setup as attached_setup
+    fibers, bodies, attached, (xs, ns, ws) = attached_setup()
+    force = [np.array([0.5, 0.0, 0.0, 0.0, 0.0, 0.3]), np.zeros(6)]
+    ts = TimeStepper(fibers, Shell(xs, ns, ws), mode="fast", tol=1e-10, bodies=bodies,
+                     body_force=force, attached=attached, graph=True)
+    S = ts.sys
+    rng = np.random.default_rng(8)
+    for _ in range(3):
+        st = ts.step()
+        assert st.residual <= 1e-10
+        for precision in ("f32c", "f64"):
+            v = torch.from_numpy(rng.normal(size=S.size)).cuda()
+            ref = S.matvec(S.precond(v), precision=precision)
+            got = ts.graphs.get(precision)(v)
+            torch.cuda.synchronize()
+            assert rel2(got.cpu().numpy(), ref.cpu().numpy()) < 1e-13, precision
+    assert 1 <= ts.graphs.captures <= 2
