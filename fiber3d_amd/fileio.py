@@ -16,6 +16,14 @@ Writing follows tstep.write_to_file (python/tstep/tstep.py:2158-2177):
   * <output>_<ID>_resume.fibers: the fiber count, then per fiber the '%s'
     header and num_points rows (x, y, z).
 Output is byte-for-byte what the reference writes for the same values.
+
+Rigid bodies (clones) follow read_input/read_clones_file.py:9-48 and the
+body part of tstep.write_to_file (tstep.py:2078-2123): a count, then one
+"x y z s p1 p2 p3" line per body (location and orientation quaternion,
+normalised on reading); trajectory frames carry a row of seven copies of the
+count and, per body, a row of seven copies of its radius before its
+location-orientation row; velocity frames a row of six copies of the count,
+then "U Omega" per body.
 """
 import numpy as np
 
@@ -106,3 +114,60 @@ def write_resume_fibers(name_file, positions, E, lengths, growing=None):
             x = np.asarray(positions[k], dtype=np.float64)
             f.write(("%s %s %s %s \n" % (x.shape[0], E[k], lengths[k], growing[k])).encode("utf-8"))
             _rows(f, x)
+
+
+def read_clones_file(name_file):
+    """(number_of_bodies, locations (n, 3), orientations (n, 4)): the
+    reference returns Quaternion objects, here their normalised entries
+    (s, p1, p2, p3)."""
+    n = None
+    loc, ori = [], []
+    with open(name_file, "r") as f:
+        for line in f:
+            line = line.split("#", 1)[0].strip()
+            if not line:
+                continue
+            data = line.split()
+            if n is None:
+                n = int(data[0])
+                continue
+            q = np.array([float(v) for v in data[3:7]])
+            loc.append([float(v) for v in data[0:3]])
+            ori.append(q / np.linalg.norm(q))
+    return n, np.array(loc), np.array(ori)
+
+
+def write_clones_frame(f, locations, orientations, radius, txt=True):
+    """One frame of <output>_<ID>.clones (txt) for the bodies of one type."""
+    locations = np.asarray(locations, np.float64).reshape(-1, 3)
+    orientations = np.asarray(orientations, np.float64).reshape(-1, 4)
+    n = locations.shape[0]
+    radius = np.broadcast_to(np.asarray(radius, np.float64), (n,))
+    if txt:
+        np.savetxt(f, np.ones((1, 7), dtype=int) * n)
+    for k in range(n):
+        row = np.concatenate((locations[k], orientations[k]))
+        if txt:
+            np.savetxt(f, np.ones((1, 7), dtype=int) * radius[k])
+            np.savetxt(f, row[None, :])
+        else:
+            f.write(("%s %s %s %s %s %s %s \n" % tuple(row)).encode("utf-8"))
+
+
+def write_body_velocities_frame(f, velocities):
+    """One frame of the bodies' velocity output: U and Omega per body."""
+    v = np.asarray(velocities, np.float64).reshape(-1, 6)
+    np.savetxt(f, np.ones((1, 6), dtype=int) * v.shape[0])
+    for k in range(v.shape[0]):
+        np.savetxt(f, v[k][None, :])
+
+
+def write_resume_clones(name_file, locations, orientations):
+    """<output>_<ID>_resume.clones for the bodies of one type."""
+    locations = np.asarray(locations, np.float64).reshape(-1, 3)
+    orientations = np.asarray(orientations, np.float64).reshape(-1, 4)
+    with open(name_file, "wb") as f:
+        f.write(("%s \n" % locations.shape[0]).encode("utf-8"))
+        for k in range(locations.shape[0]):
+            row = np.concatenate((locations[k], orientations[k]))
+            f.write(("%s %s %s %s %s %s %s \n" % tuple(row)).encode("utf-8"))

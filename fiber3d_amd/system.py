@@ -168,7 +168,7 @@ class Bodies:
     (one batched FP64 GEMV) and rotated back.  All bodies must carry the same
     node count (batched apply)."""
 
-    def __init__(self, bodies, eta, reg, eps, body_force=None):
+    def __init__(self, bodies, eta, reg, eps, body_force=None, orientations=None):
         dev = device()
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
         sizes = [np.asarray(b[1]).shape[0] for b in bodies]
@@ -188,6 +188,11 @@ class Bodies:
         self.vb = torch.zeros((self.n, 3), dtype=torch.float64, device=dev)
         # rotation of every body from the pose its inverse block was built in
         self.rot = torch.eye(3, dtype=torch.float64, device=dev).repeat(self.count, 1, 1)
+        # orientation quaternion (s, p) of the reference's bodies, advanced
+        # like body.orientation (tstep.py:1404): q <- q(Omega dt) q
+        q0 = np.tile([1.0, 0.0, 0.0, 0.0], (self.count, 1)) if orientations is None \
+            else np.asarray(orientations, float).reshape(self.count, 4)
+        self.quat = t(q0 / np.linalg.norm(q0, axis=1, keepdims=True))
         self.update_geometry()
         self.ainv = DenseGemv(torch.stack([dense_inverse(A) for A in self.pc_blocks()]))
         m = 3 * self.nb + 6
@@ -312,12 +317,12 @@ class Bodies:
             y[..., 0:1] * R[..., :, 0] + y[..., 1:2] * R[..., :, 1] + y[..., 2:3] * R[..., :, 2])
 
     def state(self):
-        """Copies of the pose (centres, nodes, normals, rotation)."""
-        return tuple(t.clone() for t in (self.centres, self.pos, self.nrm, self.rot))
+        """Copies of the pose (centres, nodes, normals, rotation, quaternion)."""
+        return tuple(t.clone() for t in (self.centres, self.pos, self.nrm, self.rot, self.quat))
 
     def set_state(self, st):
         """Restore a pose taken by state(), in place."""
-        for t, v in zip((self.centres, self.pos, self.nrm, self.rot), st):
+        for t, v in zip((self.centres, self.pos, self.nrm, self.rot, self.quat), st):
             t.copy_(v)
         self.update_geometry()
 
@@ -334,6 +339,11 @@ class Bodies:
         Rinc = c[:, :, None] * torch.eye(3, dtype=torch.float64, device=W.device) \
             + s[:, :, None] * kx + (1 - c)[:, :, None] * ax[:, :, None] * ax[:, None, :]
         self.rot.copy_(torch.bmm(Rinc, self.rot))
+        # q <- (cos(th/2), sin(th/2) axis) q, Hamilton product (quaternion.py:16-40)
+        qs, qp = torch.cos(th / 2)[:, None], torch.sin(th / 2)[:, None] * ax
+        s0, p0 = self.quat[:, :1], self.quat[:, 1:]
+        self.quat.copy_(torch.cat([qs * s0 - (qp * p0).sum(1, keepdim=True),
+                                   qs * p0 + s0 * qp + torch.cross(qp, p0, dim=1)], dim=1))
         def rot(v):
             k = ax[self.bid]
             cc, ss = c[self.bid], s[self.bid]
@@ -350,7 +360,7 @@ class FiberSystem:
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
                  lengths=None, pair_precision="f64", lu_precision="f64", lu_refine=6,
                  lu_tol=1e-13, reg=5e-3, eps=1e-5, bodies=None, body_force=None,
-                 assembly="hip", attached=None):
+                 assembly="hip", attached=None, body_orientations=None):
         self.dev = device()
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = _c0c1(epsilon, eta)
@@ -371,7 +381,8 @@ class FiberSystem:
         self.npts = int(self.offs[-1])
         self.shell = shell
         self.nper = 0 if shell is None else shell.n
-        self.bodies = Bodies(bodies, eta, reg, eps, body_force) if bodies else None
+        self.bodies = Bodies(bodies, eta, reg, eps, body_force, body_orientations) \
+            if bodies else None
         self.nbody = 0 if self.bodies is None else self.bodies.count
         self.nbn = 0 if self.bodies is None else self.bodies.n
         self.fib0 = 3 * self.nper + 3 * self.nbn + 6 * self.nbody

@@ -43,7 +43,8 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from .fileio import write_fibers_frame, write_resume_fibers
+from .fileio import (write_body_velocities_frame, write_clones_frame, write_fibers_frame,
+                     write_resume_clones, write_resume_fibers)
 from .gmres import gmres
 from .graphs import ArnoldiGraphs
 from .system import FiberSystem
@@ -83,7 +84,7 @@ class TimeStepper:
                  restart=150, maxiter=200, predictor=None, lengths=None, bodies=None,
                  body_force=None, graph=False, certify="update", record=False, lu_every=None,
                  target=None, adaptive=False, tol_tstep=1e-2, dt_min=1e-5, dt_max=1e-2,
-                 periphery_radius=None, attached=None):
+                 periphery_radius=None, attached=None, body_orientations=None):
         if mode not in ("plain", "fast"):
             raise ValueError("mode is 'plain' or 'fast'")
         if certify not in ("update", "fp64"):
@@ -117,7 +118,8 @@ class TimeStepper:
                                # a preconditioner need not be exact: the FP32
                                # factors are applied without refinement (a fixed
                                # linear map), the FP64 residual judges convergence
-                               lu_refine=0, attached=attached)
+                               lu_refine=0, attached=attached,
+                               body_orientations=body_orientations)
         self.predictor = predictor
         S = self.sys
         # recorded A P^-1 per operator precision, re-recorded after each assembly
@@ -137,6 +139,7 @@ class TimeStepper:
         self.tension = torch.zeros(S.npts, dtype=torch.float64, device=dev)
         self.shell_density = torch.zeros(3 * S.nper, dtype=torch.float64, device=dev)
         self.prev_solution = None
+        self.body_velocity = torch.zeros((S.nbody, 6), dtype=torch.float64, device=dev)
         self.history = []
 
     def inner_operator(self):
@@ -203,6 +206,8 @@ class TimeStepper:
             self.prev_solution = sol
             self.shell_density = sol[:3 * S.nper].clone()
             self.tension = sol[self.idx_t]
+            if S.bodies is not None:
+                self.body_velocity = S.bodies._vel(sol, 3 * S.nper).clone()
         else:
             # rejected (tstep.py:540-579): back to the configuration it left
             self.rejected += 1
@@ -238,6 +243,26 @@ class TimeStepper:
         """The reference's <output>_<ID>_resume.fibers for the current fibers."""
         X, _ = self.fiber_arrays()
         write_resume_fibers(name_file, X, self.sys.E, self.sys.lengths_h)
+
+    def write_bodies(self, f_clones, f_velocities=None, radius=None, txt=True):
+        """Append the bodies' locations and orientations (and, with
+        f_velocities, their last velocities) as one frame of the reference's
+        clones output (tstep.py:2088-2121).  radius defaults to each body's
+        mean node distance from its centre."""
+        B = self.sys.bodies
+        if radius is None:
+            r = torch.linalg.norm(B.pos - B.centres[B.bid], dim=1)
+            radius = torch.zeros(B.count, dtype=torch.float64, device=r.device) \
+                .index_add_(0, B.bid, r).cpu().numpy() / B.nb
+        write_clones_frame(f_clones, B.centres.cpu().numpy(), B.quat.cpu().numpy(), radius,
+                           txt=txt)
+        if f_velocities is not None:
+            write_body_velocities_frame(f_velocities, self.body_velocity.cpu().numpy())
+
+    def write_resume_bodies(self, name_file):
+        """The reference's <output>_<ID>_resume.clones for the current bodies."""
+        B = self.sys.bodies
+        write_resume_clones(name_file, B.centres.cpu().numpy(), B.quat.cpu().numpy())
 
     def verify(self):
         """True relative residuals ||b - A x|| / ||b|| of the recorded steps in

@@ -68,3 +68,27 @@ def test_trajectory_frame_layouts():
     lines = buf.getvalue().decode().splitlines()
     assert lines[0] == "2" and lines[1] == "2 0.01 1.0 0 "
     assert len(lines) == 1 + (1 + 2) + (1 + 3)
+
+
+def test_clones_round_trip_and_frames(tmp_path):
+    from fiber3d_amd.fileio import (read_clones_file, write_body_velocities_frame,
+                                    write_clones_frame, write_resume_clones)
+    loc = np.array([[0.5, -1.0, 2.0], [3.0, 0.25, -0.125]])
+    q = np.array([[1.0, 0.0, 0.0, 0.0], [0.5, 0.5, -0.5, 0.5]])
+    name = tmp_path / "run_bodies_resume.clones"
+    write_resume_clones(name, loc, q)
+    assert name.read_text().splitlines()[:2] == ["2 ", "0.5 -1.0 2.0 1.0 0.0 0.0 0.0 "]
+    n, l2, q2 = read_clones_file(name)
+    assert n == 2 and np.array_equal(l2, loc) and np.array_equal(q2, q)
+    # orientations are normalised on reading, as the reference does
+    (tmp_path / "u.clones").write_text("1\n0 0 0 2 0 0 0 # comment\n")
+    assert np.array_equal(read_clones_file(tmp_path / "u.clones")[2], [[1.0, 0, 0, 0]])
+    buf = io.BytesIO()
+    write_clones_frame(buf, loc, q, 0.5, txt=True)
+    rows = np.loadtxt(io.StringIO(buf.getvalue().decode()))
+    assert rows.shape == (5, 7) and np.all(rows[0] == 2) and np.all(rows[1] == 0.5)
+    assert np.array_equal(rows[2], np.concatenate([loc[0], q[0]]))
+    buf = io.BytesIO()
+    write_body_velocities_frame(buf, np.arange(12.0).reshape(2, 6))
+    rows = np.loadtxt(io.StringIO(buf.getvalue().decode()))
+    assert rows.shape == (3, 6) and np.all(rows[0] == 2) and np.array_equal(rows[2], np.arange(6.0, 12.0))
