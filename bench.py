@@ -137,6 +137,7 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
                 "max_true_residual": float(max(true)),
                 "max_residual_bound": float(max(st.residual_bound for st in stats)),
                 "closing_residual": {k: kinds.count(k) for k in sorted(set(kinds))},
+                "inner_rel_err": ts.inner_rel_err,
                 "verify_rhs_max_diff": drhs})
         ts.records = []
         if pred is not None and pred.use_net:

@@ -36,6 +36,12 @@ the FP64 residual is computed as usual.  From a zero guess ||H y|| ~ ||b||,
 so the bound never passes and the FP64 product is kept; from a close guess
 ||H y|| ~ ||b - A x0||, and one FP64 product per solve (the guess's residual)
 remains.
+
+Whenever an inner cycle ends with the FP64 residual instead, the same update
+is set beside it: their difference is the inner operator's error on that
+cycle's correction, ||(A_inner - A) z||, and `measure_inner=True` returns the
+largest such ratio to ||H y|| as `inner_err`, the quantity `inner_rel_err`
+must bound.
 """
 import math
 from dataclasses import dataclass
@@ -56,17 +62,19 @@ class GmresResult:
     inner_cycles: int = 0    # restart cycles run with the inner operator
     residual_kind: str = "fp64"  # "fp64": b - A x recomputed; "update": FP64-anchored update
     residual_bound: float = 0.0  # bound on the true relative residual
+    inner_err: float = None      # measured ||(A_inner - A) z|| / ||A_inner z|| (max over cycles)
 
 
 def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, inner=None,
           callback=None, fused_inner=None, fused_matvec=None, update_residual=False,
-          inner_rel_err=1e-6, target=None):
+          inner_rel_err=1e-6, target=None, measure_inner=False):
     # iterations continue to `target` (<= tol) once started; a guess (or any
     # restart) whose FP64 residual is already <= tol is accepted as it is
     t_it = tol if target is None else min(float(target), tol)
     Pinv = precond if precond is not None else (lambda v, out=None: v.clone())
     op = inner if inner is not None else matvec
     inner_cycles = 0
+    measured = None
     n = b.numel()
     dev = b.device
     bnorm = float(torch.linalg.norm(b))
@@ -94,7 +102,7 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
         hist.append(beta / bnorm)
         if beta <= tol * bnorm:
             return GmresResult(x, 0, total, restarts, beta / bnorm, guess_used, hist,
-                               inner_cycles, "fp64", beta / bnorm)
+                               inner_cycles, "fp64", beta / bnorm, measured)
         V[0] = r / beta
         H = np.zeros((m + 1, m))
         Hraw = np.zeros((m + 1, m))  # unrotated Hessenberg: A P^-1 V_k = V_{k+1} Hraw
@@ -144,21 +152,29 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
         upd = torch.mv(V[:j_done].t(), torch.from_numpy(yk).to(dev))
         x += Pinv(upd)
         restarts += 1
-        if update_residual and op is not matvec and j_done and est <= t_it:
+        r_upd = None
+        if (update_residual or measure_inner) and op is not matvec and j_done:
             hy = Hraw[:j_done + 1, :j_done] @ yk
+            hyn = float(np.linalg.norm(hy))
             k1 = j_done + 1 if hy[j_done] != 0.0 else j_done  # V[j_done] unset at breakdown
             r_upd = r - torch.mv(V[:k1].t(), torch.from_numpy(hy[:k1]).to(dev))
             rn_upd = float(torch.linalg.norm(r_upd))
-            bound = rn_upd + inner_rel_err * float(np.linalg.norm(hy))
-            if bound <= tol * bnorm:
+            bound = rn_upd + inner_rel_err * hyn
+            if update_residual and est <= t_it and bound <= tol * bnorm:
                 hist.append(rn_upd / bnorm)
                 return GmresResult(x, 0, total, restarts, rn_upd / bnorm, guess_used, hist,
-                                   inner_cycles + 1, "update", bound / bnorm)
+                                   inner_cycles + 1, "update", bound / bnorm, measured)
         r = b - matvec(x)
+        if r_upd is not None and hyn > 0.0:
+            # r - r_upd = A_inner z and r_old - r = A z for this cycle's
+            # correction z, so their difference is the inner operator's error
+            # on z, measured at no extra cost
+            e = float(torch.linalg.norm(r - r_upd)) / hyn
+            measured = e if measured is None else max(measured, e)
         if op is not matvec:
             inner_cycles += 1
             if est <= t_it and float(torch.linalg.norm(r)) > tol * bnorm:
                 op = matvec
     rn = float(torch.linalg.norm(r)) / bnorm
     return GmresResult(x, 0 if rn <= tol else total, total, restarts, rn, guess_used, hist,
-                       inner_cycles, "fp64", rn)
+                       inner_cycles, "fp64", rn, measured)

@@ -360,7 +360,7 @@ class FiberSystem:
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
                  lengths=None, pair_precision="f64", lu_precision="f64", lu_refine=6,
                  lu_tol=1e-13, reg=5e-3, eps=1e-5, bodies=None, body_force=None,
-                 assembly="hip", attached=None, body_orientations=None):
+                 assembly="hip", attached=None, body_orientations=None, hinged=None):
         self.dev = device()
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = _c0c1(epsilon, eta)
@@ -440,6 +440,7 @@ class FiberSystem:
                 "length": self.lengths[bk["ids_t"]].contiguous(),
             }
         self._setup_attached(attached)
+        self._setup_hinged(hinged)
         # work buffers
         z = lambda n: torch.zeros(n, dtype=torch.float64, device=self.dev)
         self.f_soa = z(3 * self.npts)
@@ -480,6 +481,61 @@ class FiberSystem:
                 "a_off": ti(self.bs.a_off_h[fib]),
                 "row0": ti(self.fib0 + 4 * self.offs[fib] + 4 * N - 14),
             }
+
+    def _setup_hinged(self, hinged):
+        """Fibers whose plus end is held in place, free to rotate, as a fiber
+        that reached the periphery (tstep.py:886-888)."""
+        self.hng = {}
+        if not hinged:
+            return
+        hk = np.unique(np.asarray(hinged, dtype=np.int64))
+        if hk.min() < 0 or hk.max() >= self.nfib:
+            raise ValueError("hinged: fiber index out of range")
+        ti = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int64)).to(self.dev)
+        for N, bk in self.buckets.items():
+            fib = hk[np.isin(hk, bk["ids"])]
+            if fib.size == 0:
+                continue
+            self.hng[N] = {
+                "fib": fib, "pib": ti(np.searchsorted(bk["ids"], fib)),
+                "nodeL": ti(self.offs[fib + 1] - 1), "a_off": ti(self.bs.a_off_h[fib]),
+                "row0": ti(self.fib0 + 4 * self.offs[fib] + 4 * N - 7),
+            }
+
+    def _apply_hinged(self):
+        """Replace the plus-end force rows of every hinged fiber by the
+        'velocity' rows at s = L (fiber.py:940-960): x(L)/dt and the tension
+        row 6 E c0 x_ss(L).D3 X + 2 c0 D1 T, right-hand side
+        -(v(L) + 2 c0 f(L)).x_s(L); the torque rows stay."""
+        dt, E, c0 = self.dt, self.E, self.c0
+        for N, g in self.hng.items():
+            node, X, xs, xss, D1, P, B = self._ctx[N]
+            bk = self.buckets[N]
+            sc = 2.0 / self.lengths[bk["ids_t"][g["pib"]]]
+            d1 = bk["D0"][0][-1][None, :] * sc[:, None]
+            d3 = bk["D0"][2][-1][None, :] * sc[:, None] ** 3
+            xsL, xssL = xs[g["pib"], -1], xss[g["pib"], -1]
+            na, n = len(g["fib"]), 4 * N
+            R = torch.zeros((na, 4, n), dtype=torch.float64, device=self.dev)
+            for a in range(3):
+                R[:, a, a * N + N - 1] = 1.0 / dt
+                R[:, 3, a * N:(a + 1) * N] = 6.0 * E * c0 * xssL[:, a, None] * d3
+            R[:, 3, 3 * N:] = 2.0 * c0 * d1
+            rows = torch.arange(n - 7, n - 3, device=self.dev)
+            cols = torch.arange(n, device=self.dev)
+            idx = g["a_off"][:, None, None] + cols[None, None, :] * n + rows[None, :, None]
+            self.A_buf[idx.reshape(-1)] = R.reshape(-1)
+            r0 = g["row0"]
+            XL = self.X[g["nodeL"]]
+            vL, fL = self.v_explicit[g["nodeL"]], self.force[g["nodeL"]]
+            for a in range(3):
+                self.rhs[r0 + a] = XL[:, a] / dt
+            self.rhs[r0 + 3] = -((vL + 2.0 * c0 * fL) * xsL).sum(1)
+            # refilled in place: a recorded Arnoldi graph keeps this address
+            if "xsL" not in g:
+                g["xsL"] = xsL.clone()
+            else:
+                g["xsL"].copy_(xsL)
 
     def _links(self, g):
         """Current link vectors (rotated with their bodies) and directions."""
@@ -666,6 +722,8 @@ class FiberSystem:
         self._build_rhs()
         if self.att:
             self._apply_clamped()
+        if self.hng:
+            self._apply_hinged()
         if getattr(self, "lu", None) is None:
             self.lu = BatchedLU(self.bs, self.A_buf, precision=self.lu_precision,
                                 refine=self.lu_refine, tol=self.lu_tol)
@@ -798,6 +856,9 @@ class FiberSystem:
             self.gQ[c](self.Q_buf[c], self.v_soa, res, alpha=-1.0, beta=1.0)
         if self.att:
             self._link_rows(x, out, vsoa)
+        # hinged plus ends: flow term of their tension condition (tstep.py:1167-1168)
+        for N, g in self.hng.items():
+            out.index_add_(0, g["row0"] + 3, (vsoa[g["nodeL"]] * g["xsL"]).sum(1))
         return out
 
     def precond(self, x, out=None):

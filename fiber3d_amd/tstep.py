@@ -4,9 +4,8 @@ One step follows tstep.time_step_hydro (python/tstep/tstep.py:628-1216):
 assemble (FiberSystem.prepare), solve with right-preconditioned GMRES to the
 relative tolerance, then update fiber positions, tensions and the shell
 density from the solution (tstep.py:1389-1397).  Free rigid bodies move
-rigidly with their solved velocities.  Adaptive step rejection, dynamic
-instability and output writing of the reference driver are not part of this
-path.
+rigidly with their solved velocities.  The dynamic-instability machinery of
+the reference driver is not part of this path.
 
 Two modes:
   'plain'  : FP64 pair sums, FP64 fiber LU, zero initial guess (the reference
@@ -18,7 +17,9 @@ Two modes:
 Fibers may be clamped to rigid bodies (attached=[(fiber, body, link)],
 tstep.py:878-881 with tstep_utils.build_link_matrix :948-1091): their minus
 end follows the body and its tangent turns with it, and their end force and
-moment act on the body.
+moment act on the body.  Fibers listed in hinged hold their plus end in
+place, free to rotate, as a fiber that reached the periphery
+(tstep.py:886-888).
 
 Time steps are fixed by default.  adaptive=True follows the reference's
 accept/reject scheme (tstep.py:427-581, check_error :1874-1925): after each
@@ -32,8 +33,9 @@ dt_min, where it is accepted).
 In fast mode the closing residual is, by default (certify="update"), the
 guess's FP64 residual updated by the correction's image under the
 FP32-compensated operator, accepted only under a bound that includes that
-operator's error (fiber3d_amd.gmres); certify="fp64" recomputes b - A x in
-FP64 instead.  With record=True every step keeps its geometry, right-hand
+operator's error (fiber3d_amd.gmres), an error measured on the system's own
+corrections (calibrate_inner); certify="fp64" recomputes b - A x in FP64
+instead.  With record=True every step keeps its geometry, right-hand
 side and solution, and verify() re-assembles each recorded step and returns
 the true FP64 residuals.
 """
@@ -84,7 +86,7 @@ class TimeStepper:
                  restart=150, maxiter=200, predictor=None, lengths=None, bodies=None,
                  body_force=None, graph=False, certify="update", record=False, lu_every=None,
                  target=None, adaptive=False, tol_tstep=1e-2, dt_min=1e-5, dt_max=1e-2,
-                 periphery_radius=None, attached=None, body_orientations=None):
+                 periphery_radius=None, attached=None, body_orientations=None, hinged=None):
         if mode not in ("plain", "fast"):
             raise ValueError("mode is 'plain' or 'fast'")
         if certify not in ("update", "fp64"):
@@ -98,6 +100,20 @@ class TimeStepper:
         self.lu_every = max(1, int(lu_every if lu_every is not None
                                    else (8 if mode == "fast" else 1)))
         self.target = target
+        # Relative error of the FP32-compensated operator on a step's
+        # correction, which the updated-residual certificate adds to its
+        # bound.  It depends on the geometry and the state (targets close to
+        # a surface, large cancelling forces of a fiber held at both ends),
+        # so it is measured on the corrections of guess-started solves: free
+        # of cost whenever a solve closes with the FP64 residual, and by
+        # calibrate_inner when no solve has for lu_every steps.  The bound
+        # uses the largest ratio of the last 2 lu_every steps times a safety
+        # factor, never below the floor; until the first measurement every
+        # residual is recomputed in FP64.
+        self.inner_rel_err = None
+        self.inner_rel_err_floor = 1e-6
+        self.inner_rel_err_safety = 10.0
+        self._inner_meas = []    # (step index, measured ratio)
         self.adaptive = adaptive
         self.tol_tstep, self.dt_min, self.dt_max = tol_tstep, dt_min, dt_max
         self.periphery_radius = periphery_radius
@@ -119,7 +135,7 @@ class TimeStepper:
                                # factors are applied without refinement (a fixed
                                # linear map), the FP64 residual judges convergence
                                lu_refine=0, attached=attached,
-                               body_orientations=body_orientations)
+                               body_orientations=body_orientations, hinged=hinged)
         self.predictor = predictor
         S = self.sys
         # recorded A P^-1 per operator precision, re-recorded after each assembly
@@ -148,6 +164,25 @@ class TimeStepper:
             return lambda v, out=None: S.matvec(v, out=out, precision="f32c")
         return None
 
+    def _note_inner(self, e):
+        self._inner_meas.append((self.nsteps, float(e)))
+        self._inner_meas = [(k, v) for k, v in self._inner_meas
+                            if k > self.nsteps - 2 * self.lu_every]
+        worst = max(v for _, v in self._inner_meas)
+        self.inner_rel_err = max(self.inner_rel_err_floor, self.inner_rel_err_safety * worst)
+
+    def calibrate_inner(self, z):
+        """Measure ||(A32 - A64) z|| / ||A32 z|| of the assembled operator on
+        a correction z (two products) and fold it into inner_rel_err;
+        returns the ratio."""
+        S = self.sys
+        a64 = S.matvec(z, precision="f64")
+        a32 = S.matvec(z, precision="f32c")
+        n = float(torch.linalg.norm(a32))
+        e = float(torch.linalg.norm(a32 - a64)) / n if n > 0 else 0.0
+        self._note_inner(e)
+        return e
+
     def inextensibility_error(self):
         """max over all fiber nodes of | |x_s| - 1 | for the assembled
         configuration (tstep.check_error, tstep.py:1874-1891)."""
@@ -174,10 +209,14 @@ class TimeStepper:
             fused["fused_matvec"] = lambda v, out: G.get("f64")(v, out=out)
             if self.mode == "fast":
                 fused["fused_inner"] = lambda v, out: G.get("f32c")(v, out=out)
+        certify_update = self.mode == "fast" and self.certify == "update"
         res = gmres(S.matvec, S.rhs, S.precond, tol=self.tol, restart=self.restart,
                     maxiter=self.maxiter, x0=x0, inner=self.inner_operator(),
-                    update_residual=self.mode == "fast" and self.certify == "update",
-                    target=self.target, **fused)
+                    update_residual=certify_update and self.inner_rel_err is not None,
+                    inner_rel_err=self.inner_rel_err or 1e-6, target=self.target,
+                    measure_inner=certify_update, **fused)
+        if certify_update and res.guess_used and res.inner_err is not None:
+            self._note_inner(res.inner_err)
         sol = res.x
         body_old = S.bodies.state() if S.bodies is not None else None
         if self.record:
@@ -217,6 +256,9 @@ class TimeStepper:
         if not accept or dt_new != dt:
             S.dt = dt_new
             S.prepare(refactor=True)
+        if accept and certify_update and res.guess_used and (
+                not self._inner_meas or self.nsteps - self._inner_meas[-1][0] >= self.lu_every):
+            self.calibrate_inner(sol - x0)
         torch.cuda.synchronize()
         st = StepStats(res.iterations, res.restarts, res.residual, time.perf_counter() - t0,
                        res.guess_used, res.residual_kind, res.residual_bound, accept, dt, err)

@@ -233,7 +233,8 @@ def compute_RHS(fm, x, length, dt, c0, c1, penalty=500.0, force=None, flow=None,
 
 def apply_BC_rectangular(fm, x, length, E, A, RHS, end_force=None,
                          start_force=None, start_torque=None, end_torque=None,
-                         clamped=False, dt=None, c0=None, flow=None, force=None):
+                         clamped=False, dt=None, c0=None, flow=None, force=None,
+                         hinged=False):
     """Square system [P_cheb A; B], [P_cheb RHS; B_RHS] for force/torque-free
     ends with optional prescribed end forces/torques (fiber.py:810-961, the
     'force' and 'torque' branches).  clamped=True replaces the minus-end rows
@@ -241,7 +242,10 @@ def apply_BC_rectangular(fm, x, length, E, A, RHS, end_force=None,
     :887-898) with zero prescribed velocities, as tstep sets them for a fiber
     attached to a body (tstep.py:878-881): x(0)/dt and x_s(0)/dt rows, and
     the tension row 6 E c0 x_ss(0) . D3 X + 2 c0 D1 T whose right-hand side
-    carries -(flow(0) + 2 c0 force(0)) . x_s(0)."""
+    carries -(flow(0) + 2 c0 force(0)) . x_s(0).  hinged=True replaces the
+    plus-end force rows by the 'velocity' branch at s = L (fiber.py:940-960),
+    as for a fiber that reached the periphery (tstep.py:886-888): x(L)/dt and
+    the same tension row at L; the torque rows stay."""
     N = fm.num_points
     D1, D2, D3, _ = fm.Ds(length)
     xs, xss, _, _ = derivatives(fm, x, length)
@@ -283,6 +287,18 @@ def apply_BC_rectangular(fm, x, length, E, A, RHS, end_force=None,
         if force is not None:
             Bv[3] -= 2.0 * c0 * np.dot(np.asarray(force, float)[0], xs[0])
         Bv[4:7] = xs[0] / dt
+    if hinged:
+        B[7:11] = 0.0
+        for a in range(3):
+            B[7 + a, a * N + N - 1] = 1.0 / dt
+            B[10, a * N:(a + 1) * N] = 6.0 * E * c0 * xss[-1, a] * D3[-1]
+        B[10, 3 * N:] = 2.0 * c0 * D1[-1]
+        Bv[7:10] = x[-1] / dt
+        Bv[10] = 0.0
+        if flow is not None:
+            Bv[10] -= np.dot(np.asarray(flow, float)[-1], xs[-1])
+        if force is not None:
+            Bv[10] -= 2.0 * c0 * np.dot(np.asarray(force, float)[-1], xs[-1])
     A_sq = np.vstack((fm.P_cheb @ A, B))
     b = np.concatenate((fm.P_cheb @ RHS, Bv))
     return A_sq, b

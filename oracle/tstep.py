@@ -38,13 +38,15 @@ from . import periphery as P
 class HydroSystem:
     def __init__(self, fibers, shell=None, eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
-                 lengths=None, bodies=None, body_force=None, attached=None):
+                 lengths=None, bodies=None, body_force=None, attached=None, hinged=None):
         """fibers: list of (N_k,3) arrays; shell: (x, normals, weights) or None;
         bodies: list of (centre (3,), nodes (Nb,3), outward normals, weights);
         body_force: optional (nbody, 6) external force and torque per body;
         attached: optional list of (fiber k, body j, link (3,)): fiber k's
         minus end is clamped to body j at centre_j + link (tstep.py:878-881,
-        tstep_utils.build_link_matrix :948-1091)."""
+        tstep_utils.build_link_matrix :948-1091); hinged: fibers whose plus
+        end is held in place with free rotation, as a fiber that reached the
+        periphery (tstep.py:886-888)."""
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = OF.constants(epsilon, eta)
         self.fibers = [np.asarray(x, float) for x in fibers]
@@ -67,6 +69,7 @@ class HydroSystem:
         self.attached = [] if attached is None else [(int(k), int(j), np.asarray(l, float))
                                                      for k, j, l in attached]
         self.clamped = {k for k, _, _ in self.attached}
+        self.hinged = set() if hinged is None else {int(k) for k in hinged}
         self._setup(end_force)
 
     # -------------------------------------------------------------------
@@ -138,7 +141,8 @@ class HydroSystem:
                                flow=v_k)
             A_sq, b = OF.apply_BC_rectangular(fm, x, L, self.E, A, R, end_force=f_k[-1],
                                               clamped=k in self.clamped, dt=self.dt,
-                                              c0=self.c0, flow=v_k, force=f_k)
+                                              c0=self.c0, flow=v_k, force=f_k,
+                                              hinged=k in self.hinged)
             self.A_sq.append(A_sq)
             o = self.fib0 + 4 * self.offs[k]
             rhs[o:o + 4 * N] = b
@@ -316,6 +320,12 @@ class HydroSystem:
             res[r0:r0 + 3] -= vU
             res[r0 + 3] += -np.dot(vU, xs0) + np.dot(v_fib[self.offs[k]], xs0)
             res[r0 + 4:r0 + 7] -= np.cross(W, link / np.linalg.norm(link))
+        # hinged plus ends: the flow term of their tension condition
+        # (tstep.py:1167-1168)
+        for k in self.hinged:
+            N = self.sizes[k]
+            rt = self.fib0 + 4 * self.offs[k] + 4 * N - 4
+            res[rt] += np.dot(v_fib[self.offs[k + 1] - 1], self.derivs[k][0][-1])
         return res
 
     def precond(self, x):

@@ -9,7 +9,7 @@ fibers, the inputs and the reference outputs of:
   fiber.form_linear_operator                      (fiber/fiber.py:322-447)
   fiber.compute_RHS                               (fiber/fiber.py:473-582)
   fiber.apply_BC_rectangular                      (fiber/fiber.py:810-961),
-                                                  free and clamped minus end
+                                                  free, clamped minus end, hinged plus end
   fiber.force_operator                            (fiber/fiber.py:1399-1431)
 Only arrays are stored (no pickles); the reference itself is not copied.
 """
@@ -51,6 +51,9 @@ def main():
         fib.set_BC(BC_start_0="velocity", BC_start_1="angular_velocity",
                    BC_end_vec_0=force[-1])
         A_cl, b_cl = fib.apply_BC_rectangular(A, RHS, fm, flow, force)
+        # plus end hinged (a fiber that reached the periphery, tstep.py:886-888)
+        fib.set_BC(BC_end_0="velocity", BC_end_1="torque")
+        A_hg, b_hg = fib.apply_BC_rectangular(A, RHS, fm, flow, force)
         F = fib.force_operator(fm)
         w, _, _, _ = fm.get_matrices(L, N, "weights_all")
         _, _, P_cheb, _ = fm.get_matrices(L, N, "PX_PT_Pcheb")
@@ -61,7 +64,8 @@ def main():
                      p + "D1": D1, p + "D2": D2, p + "D3": D3, p + "D4": D4,
                      p + "weights": w, p + "P_cheb": P_cheb, p + "A": A, p + "RHS": RHS,
                      p + "A_sq": A_sq, p + "b": b, p + "F": F,
-                     p + "A_sq_clamped": A_cl, p + "b_clamped": b_cl})
+                     p + "A_sq_clamped": A_cl, p + "b_clamped": b_cl,
+                     p + "A_sq_hinged": A_hg, p + "b_hinged": b_hg})
     np.savez_compressed(OUT, **data)
     print(f"wrote {OUT} ({os.path.getsize(OUT) / 1e3:.0f} kB)")
 

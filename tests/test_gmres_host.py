@@ -152,6 +152,27 @@ def test_residual_update_rejected_when_bound_fails():
     assert np.linalg.norm(b - A @ res.x.numpy()) / np.linalg.norm(b) <= 1.01e-10
 
 
+def test_inner_error_measured_at_fp64_closing():
+    """measure_inner: a cycle that closes with the FP64 residual reports the
+    inner operator's error on its own correction z = x - x0,
+    ||(A_inner - A) z|| / ||A_inner z||, recomputed here from the matrices."""
+    A, b, mv, pc, bt = _system(seed=10)
+    rng = np.random.default_rng(13)
+    E = rng.normal(size=A.shape) * 1e-8 * np.abs(A).max()
+    Ai = torch.from_numpy(A + E)
+    inner = lambda v, out=None: torch.mv(Ai, v, out=out) if out is not None else torch.mv(Ai, v)
+    x = np.linalg.solve(A, b)
+    x0 = x + 1e-5 * rng.normal(size=x.shape)
+    res = gmres(mv, bt, pc, tol=1e-10, x0=torch.from_numpy(x0), inner=inner,
+                measure_inner=True)
+    assert res.residual_kind == "fp64" and res.inner_cycles == 1 and res.restarts == 1
+    z = res.x.numpy() - x0
+    ref = np.linalg.norm(E @ z) / np.linalg.norm((A + E) @ z)
+    assert abs(res.inner_err - ref) <= 1e-4 * ref
+    # without measure_inner (and no update) nothing is measured
+    assert gmres(mv, bt, pc, tol=1e-10, x0=torch.from_numpy(x0), inner=inner).inner_err is None
+
+
 def test_target_below_tolerance():
     """Once iterating, GMRES continues to `target`; a guess whose residual
     is already within tol is accepted with no iteration."""

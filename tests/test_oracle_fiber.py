@@ -52,6 +52,12 @@ def test_fiber_assembly_matches_reference_golden(gold, ci):
     # only the seven minus-end rows differ from the free fiber
     assert np.array_equal(A_cl[:4 * N - 14], A_sq[:4 * N - 14])
     assert np.array_equal(A_cl[4 * N - 7:], A_sq[4 * N - 7:])
+    # plus end hinged at the periphery: 'velocity' + 'torque' rows at s = L,
+    # no prescribed end force
+    A_hg, b_hg = OF.apply_BC_rectangular(fm, x, L, E, A, R, hinged=True, dt=dt, c0=c0,
+                                         flow=g["flow"], force=g["force"])
+    assert rel2(A_hg, g["A_sq_hinged"]) < 1e-13
+    assert rel2(b_hg, g["b_hinged"]) < 1e-13
 
 
 @pytest.mark.parametrize("N", [16, 32, 64])

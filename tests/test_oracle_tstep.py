@@ -4,6 +4,7 @@ import pytest
 
 from conftest import rel2
 from fiber3d_amd import configs
+from oracle import fiber as OF
 from oracle import tstep as OT
 
 
@@ -150,3 +151,52 @@ def test_clamped_fiber_torque_turns_the_assembly():
     assert tip_v[1] < 0.0
     # the tip, three radii out, lags the rigid rotation a little (bending)
     assert np.linalg.norm(tip_v - U - np.cross(W, fiber[-1])) < 0.15 * np.linalg.norm(tip_v)
+
+
+def _hinged_step(F, centre=(0.0, 0.0, -0.5), hinged=True, E=10.0, dt=0.02, N=24):
+    """A straight fiber along z on [0, 1] with its plus end (z = 1) hinged
+    (tstep.py:886-888) and a sphere of radius 0.3 at centre pushed by F; one
+    implicit step solved to 1e-10 -> (system, new positions, solution)."""
+    body = configs.make_sphere_body(list(centre), 0.3, 300)
+    s = np.linspace(0.0, 1.0, N)
+    fiber = np.stack([np.zeros(N), np.zeros(N), s], axis=1)
+    S = OT.HydroSystem([fiber], None, eta=1.0, dt=dt, E=E, end_force=0.0, bodies=[body],
+                       body_force=[np.concatenate([F, np.zeros(3)])],
+                       hinged=[0] if hinged else None)
+    x, info, _ = OT.gmres(S.matvec, S.rhs, S.precond, tol=1e-10)
+    assert info == 0
+    Xn = x[S.fib0:S.fib0 + 3 * N].reshape(3, N).T
+    return S, fiber, Xn, x
+
+
+def test_hinged_plus_end_holds_and_the_fiber_swings_about_it():
+    """A sideways push: the hinged end stays put while the rest of the fiber
+    is carried along +x, less and less towards the hinge, and the free fiber
+    of the same set-up is carried along at its plus end too."""
+    F = np.array([2.0, 0.0, 0.0])
+    S, X0, Xn, _ = _hinged_step(F)
+    v = (Xn - X0) / S.dt
+    assert np.linalg.norm(v[-1]) < 1e-7
+    assert v[0, 0] > 0.0 and np.all(np.diff(v[:, 0]) < 0.0)
+    _, _, Xf, _ = _hinged_step(F, hinged=False)
+    assert (Xf[-1, 0] - X0[-1, 0]) / S.dt > 0.1 * v[0, 0]
+
+
+def test_hinged_end_holds_the_fiber_against_an_axial_flow():
+    """A sphere beyond the hinge pushed along the fiber: an inextensible
+    fiber with a fixed end cannot follow the flow (the free fiber of the
+    same set-up is carried along), so it stays put, in tension.  Keeping
+    |x_s| = 1 near the hinge needs the tangential flow there in the hinge's
+    tension row (tstep.py:1167-1168): without it the step stretches the
+    fiber by about 2e-4."""
+    F, c = np.array([0.0, 0.0, -3.0]), (0.0, 0.0, 1.5)
+    S, X0, Xn, x = _hinged_step(F, c)
+    N = X0.shape[0]
+    _, _, Xf, _ = _hinged_step(F, c, hinged=False)
+    free = np.abs(Xf - X0).max()
+    assert free > 1e-3
+    assert np.abs(Xn - X0).max() < 1e-2 * free
+    xs = OF.derivatives(S.fm[N], Xn, 1.0)[0]
+    assert np.abs(np.linalg.norm(xs, axis=1) - 1.0).max() < 1e-6
+    T = x[S.fib0 + 3 * N:S.fib0 + 4 * N]
+    assert T[-1] > 0.1
