@@ -17,6 +17,9 @@ Writing follows tstep.write_to_file (python/tstep/tstep.py:2158-2177):
     header and num_points rows (x, y, z).
 Output is byte-for-byte what the reference writes for the same values.
 
+The simulation input file (options, structures) follows
+read_input/read_input.py:9-191 (ReadInput).
+
 Rigid bodies (clones) follow read_input/read_clones_file.py:9-48 and the
 body part of tstep.write_to_file (tstep.py:2078-2123): a count, then one
 "x y z s p1 p2 p3" line per body (location and orientation quaternion,
@@ -25,7 +28,127 @@ count and, per body, a row of seven copies of its radius before its
 location-orientation row; velocity frames a row of six copies of the count,
 then "U Omega" per body.
 """
+import ntpath
+
 import numpy as np
+
+
+def _floats(text):
+    """Whitespace-separated numbers up to the first token that is not one
+    (what numpy.fromstring(text, sep=' ') returns, without its warning)."""
+    out = []
+    for tok in text.split():
+        try:
+            out.append(float(tok))
+        except ValueError:
+            break
+    return np.array(out)
+
+
+def _number(options, key, default, kind):
+    # an absent option and an empty value both take the default, as the
+    # reference's `options.get(key) or default`
+    v = options.get(key)
+    return kind(v) if v else kind(default)
+
+
+class ReadInput:
+    """The simulation input file (read_input/read_input.py:9-191).
+
+    One option per line, "name value...", '#' starts a comment, blank lines
+    are skipped; the value is the rest of the line.  Repeated structure
+    options are numbered in order (structure0, structure1, ...; likewise
+    structure_fibers, structure_body_fibers, mm_surface_structure).  The
+    attributes below carry the reference's names and defaults."""
+
+    _FLOATS = [("dt", 0.0), ("eta", 1.0), ("kT", 1.0), ("solver_tolerance", 1e-08),
+               ("E", 0.0), ("length_fibers", 0.0), ("epsilon", 1e-03), ("v_growth", 0.0),
+               ("v_shrink", 0.0), ("rate_catastrophe", 0.0), ("rate_rescue", 0.0),
+               ("rate_seed", 0.0), ("force_stall", 1.0), ("rate_catastrophe_stall", 1.0),
+               ("length_min", 0.0), ("molecular_motor_radius", 1.0),
+               ("molecular_motor_speed", 1.0), ("molecular_motor_force_stall", 1.0),
+               ("molecular_motor_spring_constant", 1.0), ("molecular_motor_rest_length", 1.0),
+               ("molecular_motor_bind_frequency", 1.0),
+               ("molecular_motor_unbind_frequency", 1.0), ("molecular_motor_kernel_sigma", 1.0)]
+    _INTS = [("n_steps", 0), ("initial_step", 0), ("n_save", 1), ("num_points_fibers", 0),
+             ("p_fmm", 8), ("num_points_finite_diff", 0)]
+    _STRINGS = [("scheme", "deterministic_forward_euler"), ("output_name", "run"),
+                ("save_clones", "one_file_per_step"),
+                ("mobility_vector_prod_implementation", "single_layer_double_layer")]
+    _RAW = ["random_state", "seed", "force_file", "velocity_file"]
+    _OR_NONE = ["force_generator_config", "molecular_motor_config",
+                "molecular_motor_attached_ends", "molecular_motor_head_config",
+                "molecular_motor_base_config", "molecular_motor_s_head",
+                "molecular_motor_s_base"]
+    _NUMBERED = ("structure", "structure_fibers", "structure_body_fibers",
+                 "mm_surface_structure")
+
+    def __init__(self, entries):
+        self.entries = entries
+        self.input_file = entries
+        self.options = {}
+        count = {k: 0 for k in self._NUMBERED}
+        with open(self.input_file, "r") as f:
+            for line in f:
+                line = line.split("#", 1)[0].strip()
+                if not line:
+                    continue
+                parts = line.split(None, 1)
+                if len(parts) < 2:
+                    # the reference's tuple unpacking fails on a bare name
+                    raise ValueError("option %r has no value" % parts[0])
+                option, value = parts
+                if option in count:
+                    n = count[option]
+                    count[option] = n + 1
+                    option = option + str(n)
+                self.options[option] = value
+        o = self.options
+        for key, d in self._FLOATS:
+            setattr(self, key, _number(o, key, d, float))
+        for key, d in self._INTS:
+            setattr(self, key, _number(o, key, d, int))
+        for key, d in self._STRINGS:
+            setattr(self, key, str(o.get(key) or d))
+        for key in self._RAW:
+            setattr(self, key, o.get(key))
+        for key in self._OR_NONE:
+            setattr(self, key, o.get(key) or None)
+        self.periodic_length = _floats(o.get("periodic_length") or "0 0 0")
+        self.plot_velocity_field = _floats(o.get("plot_velocity_field") or "None")
+
+        def numbered(name, n, split=True):
+            vals = [o.get(name + str(i)) for i in range(n)]
+            return [str(v).split() if split else str(v) for v in vals]
+
+        def ids(structs, pick):
+            # file name without directory and without its 7-character suffix
+            # ('.clones' / '.fibers')
+            return [ntpath.split(pick(s))[1][:-7] for s in structs]
+
+        self.structures = numbered("structure", count["structure"])
+        self.structures_ID = ids(self.structures, lambda s: s[1])
+        if self.initial_step > 0:
+            for k, st in enumerate(self.structures):
+                st[1] = "%s.%s.%s.clones" % (self.output_name, self.structures_ID[k],
+                                             str(self.initial_step).zfill(8))
+        self.structures_fibers = numbered("structure_fibers", count["structure_fibers"],
+                                          split=False)
+        self.structures_fibers_ID = ids(self.structures_fibers, lambda s: s)
+        if self.initial_step > 0 and self.structures_fibers:
+            # read_input.py:136-139 assigns into a string here, which raises
+            raise TypeError("restarting fiber structures (initial_step > 0) is not supported "
+                            "by the reference's input reader")
+        self.structures_body_fibers = numbered("structure_body_fibers",
+                                               count["structure_body_fibers"])
+        self.structures_body_fibers_ID = ids(self.structures_body_fibers, lambda s: s[1])
+        if self.initial_step > 0:
+            # read_input.py:162-164: prints this and exits (status 0)
+            print("ERROR, code is not ready to restart from old configurations")
+            raise SystemExit()
+        self.mm_surface_structure = numbered("mm_surface_structure",
+                                             count["mm_surface_structure"])
+        self.mm_surface_structure_ID = ids(self.mm_surface_structure, lambda s: s[0])
 
 
 def read_fibers_file(name_file):

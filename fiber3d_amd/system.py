@@ -360,7 +360,8 @@ class FiberSystem:
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
                  lengths=None, pair_precision="f64", lu_precision="f64", lu_refine=6,
                  lu_tol=1e-13, reg=5e-3, eps=1e-5, bodies=None, body_force=None,
-                 assembly="hip", attached=None, body_orientations=None, hinged=None):
+                 assembly="hip", attached=None, body_orientations=None, hinged=None,
+                 cortex=False):
         self.dev = device()
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = _c0c1(epsilon, eta)
@@ -440,6 +441,7 @@ class FiberSystem:
                 "length": self.lengths[bk["ids_t"]].contiguous(),
             }
         self._setup_attached(attached)
+        self._setup_cortex(cortex)
         self._setup_hinged(hinged)
         # work buffers
         z = lambda n: torch.zeros(n, dtype=torch.float64, device=self.dev)
@@ -479,6 +481,27 @@ class FiberSystem:
                 "lref": torch.from_numpy(lref[sel]).to(self.dev),
                 "node0": ti(self.offs[fib]), "xo": ti(4 * self.offs[fib]),
                 "a_off": ti(self.bs.a_off_h[fib]),
+                "row0": ti(self.fib0 + 4 * self.offs[fib] + 4 * N - 14),
+            }
+
+    def _setup_cortex(self, cortex):
+        """attached_to_cortex (tstep.py:881-883): every fiber not attached to
+        a body has its minus end clamped in place, the same rows as a fiber
+        clamped to a body at rest, with no link."""
+        self.cor = {}
+        if not cortex:
+            return
+        taken = set()
+        for g in self.att.values():
+            taken |= set(g["fib"].tolist())
+        ti = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int64)).to(self.dev)
+        for N, bk in self.buckets.items():
+            fib = np.array([k for k in bk["ids"] if k not in taken], dtype=np.int64)
+            if fib.size == 0:
+                continue
+            self.cor[N] = {
+                "fib": fib, "pib": ti(np.searchsorted(bk["ids"], fib)),
+                "node0": ti(self.offs[fib]), "a_off": ti(self.bs.a_off_h[fib]),
                 "row0": ti(self.fib0 + 4 * self.offs[fib] + 4 * N - 14),
             }
 
@@ -543,12 +566,12 @@ class FiberSystem:
         return link, link / torch.linalg.norm(link, dim=1, keepdim=True)
 
     def _apply_clamped(self):
-        """Replace the minus-end boundary rows of every attached fiber by the
-        'velocity' and 'angular_velocity' rows (fiber.py:855-877, :887-898):
+        """Replace the minus-end boundary rows of every fiber clamped to a
+        body or to the cortex by the 'velocity' and 'angular_velocity' rows (fiber.py:855-877, :887-898):
         x(0)/dt, x_s(0)/dt, and the tension row 6 E c0 x_ss(0).D3 X +
         2 c0 D1 T with right-hand side -(v(0) + 2 c0 f(0)).x_s(0)."""
         dt, E, c0 = self.dt, self.E, self.c0
-        for N, g in self.att.items():
+        for N, g in list(self.att.items()) + list(self.cor.items()):
             node, X, xs, xss, D1, P, B = self._ctx[N]
             bk = self.buckets[N]
             sc = 2.0 / self.lengths[bk["ids_t"][g["pib"]]]
@@ -720,7 +743,7 @@ class FiberSystem:
             self.xs[node.reshape(-1)] = xs.reshape(-1, 3)
             self._ctx[N] = (node, X, xs, xss, D1, P, B)
         self._build_rhs()
-        if self.att:
+        if self.att or self.cor:
             self._apply_clamped()
         if self.hng:
             self._apply_hinged()
@@ -856,6 +879,10 @@ class FiberSystem:
             self.gQ[c](self.Q_buf[c], self.v_soa, res, alpha=-1.0, beta=1.0)
         if self.att:
             self._link_rows(x, out, vsoa)
+        # minus ends clamped to the cortex: flow term of their tension
+        # condition (tstep.py:1162-1166)
+        for N, g in self.cor.items():
+            out.index_add_(0, g["row0"] + 3, (vsoa[g["node0"]] * g["xs0"]).sum(1))
         # hinged plus ends: flow term of their tension condition (tstep.py:1167-1168)
         for N, g in self.hng.items():
             out.index_add_(0, g["row0"] + 3, (vsoa[g["nodeL"]] * g["xsL"]).sum(1))

@@ -19,7 +19,8 @@ tstep.py:878-881 with tstep_utils.build_link_matrix :948-1091): their minus
 end follows the body and its tangent turns with it, and their end force and
 moment act on the body.  Fibers listed in hinged hold their plus end in
 place, free to rotate, as a fiber that reached the periphery
-(tstep.py:886-888).
+(tstep.py:886-888); cortex=True clamps the minus end of every other fiber in
+place (attached_to_cortex, tstep.py:881-883).
 
 Time steps are fixed by default.  adaptive=True follows the reference's
 accept/reject scheme (tstep.py:427-581, check_error :1874-1925): after each
@@ -86,7 +87,8 @@ class TimeStepper:
                  restart=150, maxiter=200, predictor=None, lengths=None, bodies=None,
                  body_force=None, graph=False, certify="update", record=False, lu_every=None,
                  target=None, adaptive=False, tol_tstep=1e-2, dt_min=1e-5, dt_max=1e-2,
-                 periphery_radius=None, attached=None, body_orientations=None, hinged=None):
+                 periphery_radius=None, attached=None, body_orientations=None, hinged=None,
+                 cortex=False):
         if mode not in ("plain", "fast"):
             raise ValueError("mode is 'plain' or 'fast'")
         if certify not in ("update", "fp64"):
@@ -106,10 +108,10 @@ class TimeStepper:
         # a surface, large cancelling forces of a fiber held at both ends),
         # so it is measured on the corrections of guess-started solves: free
         # of cost whenever a solve closes with the FP64 residual, and by
-        # calibrate_inner when no solve has for lu_every steps.  The bound
-        # uses the largest ratio of the last 2 lu_every steps times a safety
-        # factor, never below the floor; until the first measurement every
-        # residual is recomputed in FP64.
+        # calibrate_inner (two products) when no solve has for
+        # 2 lu_every steps.  The bound uses the largest ratio of the last
+        # 4 lu_every steps times a safety factor, never below the floor;
+        # until the first measurement every residual is recomputed in FP64.
         self.inner_rel_err = None
         self.inner_rel_err_floor = 1e-6
         self.inner_rel_err_safety = 10.0
@@ -135,7 +137,8 @@ class TimeStepper:
                                # factors are applied without refinement (a fixed
                                # linear map), the FP64 residual judges convergence
                                lu_refine=0, attached=attached,
-                               body_orientations=body_orientations, hinged=hinged)
+                               body_orientations=body_orientations, hinged=hinged,
+                               cortex=cortex)
         self.predictor = predictor
         S = self.sys
         # recorded A P^-1 per operator precision, re-recorded after each assembly
@@ -167,7 +170,7 @@ class TimeStepper:
     def _note_inner(self, e):
         self._inner_meas.append((self.nsteps, float(e)))
         self._inner_meas = [(k, v) for k, v in self._inner_meas
-                            if k > self.nsteps - 2 * self.lu_every]
+                            if k > self.nsteps - 4 * self.lu_every]
         worst = max(v for _, v in self._inner_meas)
         self.inner_rel_err = max(self.inner_rel_err_floor, self.inner_rel_err_safety * worst)
 
@@ -257,7 +260,8 @@ class TimeStepper:
             S.dt = dt_new
             S.prepare(refactor=True)
         if accept and certify_update and res.guess_used and (
-                not self._inner_meas or self.nsteps - self._inner_meas[-1][0] >= self.lu_every):
+                not self._inner_meas
+                or self.nsteps - self._inner_meas[-1][0] >= 2 * self.lu_every):
             self.calibrate_inner(sol - x0)
         torch.cuda.synchronize()
         st = StepStats(res.iterations, res.restarts, res.residual, time.perf_counter() - t0,

@@ -38,7 +38,8 @@ from . import periphery as P
 class HydroSystem:
     def __init__(self, fibers, shell=None, eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
-                 lengths=None, bodies=None, body_force=None, attached=None, hinged=None):
+                 lengths=None, bodies=None, body_force=None, attached=None, hinged=None,
+                 cortex=False):
         """fibers: list of (N_k,3) arrays; shell: (x, normals, weights) or None;
         bodies: list of (centre (3,), nodes (Nb,3), outward normals, weights);
         body_force: optional (nbody, 6) external force and torque per body;
@@ -46,7 +47,9 @@ class HydroSystem:
         minus end is clamped to body j at centre_j + link (tstep.py:878-881,
         tstep_utils.build_link_matrix :948-1091); hinged: fibers whose plus
         end is held in place with free rotation, as a fiber that reached the
-        periphery (tstep.py:886-888)."""
+        periphery (tstep.py:886-888); cortex: every fiber not attached to a
+        body has its minus end clamped in place (attached_to_cortex,
+        tstep.py:881-883)."""
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = OF.constants(epsilon, eta)
         self.fibers = [np.asarray(x, float) for x in fibers]
@@ -69,6 +72,8 @@ class HydroSystem:
         self.attached = [] if attached is None else [(int(k), int(j), np.asarray(l, float))
                                                      for k, j, l in attached]
         self.clamped = {k for k, _, _ in self.attached}
+        self.cortex = sorted(set(range(len(self.fibers))) - self.clamped) if cortex else []
+        self.clamped |= set(self.cortex)
         self.hinged = set() if hinged is None else {int(k) for k in hinged}
         self._setup(end_force)
 
@@ -320,6 +325,10 @@ class HydroSystem:
             res[r0:r0 + 3] -= vU
             res[r0 + 3] += -np.dot(vU, xs0) + np.dot(v_fib[self.offs[k]], xs0)
             res[r0 + 4:r0 + 7] -= np.cross(W, link / np.linalg.norm(link))
+        # minus ends clamped to the cortex: only the flow term (tstep.py:1162-1166)
+        for k in self.cortex:
+            rt = self.fib0 + 4 * self.offs[k] + 4 * self.sizes[k] - 11
+            res[rt] += np.dot(v_fib[self.offs[k]], self.derivs[k][0][0])
         # hinged plus ends: the flow term of their tension condition
         # (tstep.py:1167-1168)
         for k in self.hinged:

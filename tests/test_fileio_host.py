@@ -1,10 +1,12 @@
 """Reference-format fiber files: reading the reference's own example
-structures (tests/golden/*.fibers, copied from python/multi_fibers/), and
+structures (tests/golden/*.fibers, copied from python/multi_fibers/), its
+input-file reader (ReadInput), and
 writing trajectory frames and resume files that read back exactly."""
 import io
 import os
 
 import numpy as np
+import pytest
 
 from fiber3d_amd.fileio import (fibers_from_file, read_fibers_file, write_fibers_frame,
                                 write_resume_fibers)
@@ -92,3 +94,99 @@ def test_clones_round_trip_and_frames(tmp_path):
     write_body_velocities_frame(buf, np.arange(12.0).reshape(2, 6))
     rows = np.loadtxt(io.StringIO(buf.getvalue().decode()))
     assert rows.shape == (3, 6) and np.all(rows[0] == 2) and np.array_equal(rows[2], np.arange(6.0, 12.0))
+
+
+INPUT = """# Select scheme
+scheme                                   linear_solver_fibers_and_bodies
+
+# Time step and number of steps
+dt                                       1e-02
+n_steps                                  8000
+n_save                                   20
+initial_step                             0
+eta                                      1.0     # viscosity
+num_points_fibers                        32
+E                                        1.0
+length_fibers                            5
+num_points_finite_diff                   11
+rate_catastrophe_stall                   1e+03
+# structure_fibers                       Structures/skipped.fibers
+structure_fibers                         Structures/one.fibers
+structure_fibers                         /data/run/two.fibers
+structure_body_fibers                    Structures/b.vertex  Structures/b.clones Structures/b.fibers  Structures/b.links
+structure                                sphere.shape Structures/sphere.clones
+solver_tolerance                         1e-12
+seed                                     1
+output_name                              data/run1
+periodic_length                          1 2.5 3
+plot_velocity_field                      -10 10 50 -10 10 50 -10 10 50
+"""
+
+
+def test_read_input_options_and_structures(tmp_path):
+    """ReadInput (read_input/read_input.py:9-191): values, the reference's
+    defaults for absent options, numbered structures and their IDs."""
+    from fiber3d_amd.fileio import ReadInput
+    f = tmp_path / "data.main"
+    f.write_text(INPUT)
+    r = ReadInput(str(f))
+    assert r.scheme == "linear_solver_fibers_and_bodies"
+    assert (r.dt, r.n_steps, r.n_save, r.initial_step) == (1e-2, 8000, 20, 0)
+    assert (r.eta, r.E, r.length_fibers, r.num_points_fibers) == (1.0, 1.0, 5.0, 32)
+    assert r.num_points_finite_diff == 11 and r.rate_catastrophe_stall == 1e3
+    assert r.solver_tolerance == 1e-12 and r.seed == "1" and r.random_state is None
+    assert r.output_name == "data/run1"
+    # defaults of options the file does not set
+    assert (r.kT, r.epsilon, r.force_stall, r.p_fmm) == (1.0, 1e-3, 1.0, 8)
+    assert r.save_clones == "one_file_per_step" and r.molecular_motor_config is None
+    assert r.mobility_vector_prod_implementation == "single_layer_double_layer"
+    np.testing.assert_array_equal(r.periodic_length, [1.0, 2.5, 3.0])
+    np.testing.assert_array_equal(r.plot_velocity_field, [-10, 10, 50] * 3)
+    assert r.structures_fibers == ["Structures/one.fibers", "/data/run/two.fibers"]
+    assert r.structures_fibers_ID == ["one", "two"]
+    assert r.structures_body_fibers == [["Structures/b.vertex", "Structures/b.clones",
+                                         "Structures/b.fibers", "Structures/b.links"]]
+    assert r.structures_body_fibers_ID == ["b"]
+    assert r.structures == [["sphere.shape", "Structures/sphere.clones"]]
+    assert r.structures_ID == ["sphere"]
+    assert r.mm_surface_structure == [] and r.options["structure_fibers1"] == "/data/run/two.fibers"
+
+
+def test_read_input_defaults_and_restart(tmp_path):
+    from fiber3d_amd.fileio import ReadInput
+    f = tmp_path / "empty.main"
+    f.write_text("# nothing\n\n")
+    r = ReadInput(str(f))
+    assert (r.dt, r.eta, r.n_save, r.solver_tolerance) == (0.0, 1.0, 1, 1e-8)
+    assert r.scheme == "deterministic_forward_euler" and r.output_name == "run"
+    np.testing.assert_array_equal(r.periodic_length, [0.0, 0.0, 0.0])
+    assert r.plot_velocity_field.size == 0
+    # restarting: bodies read their clones from the output directory; the
+    # reference stops there (read_input.py:113-116, :162-164)
+    f.write_text("initial_step 40\noutput_name out/run\nstructure s.vertex in/s.clones\n")
+    with pytest.raises(SystemExit):
+        ReadInput(str(f))
+    f.write_text("initial_step 40\nstructure_fibers in/f.fibers\n")
+    with pytest.raises(TypeError):
+        ReadInput(str(f))
+    f.write_text("dt\n")
+    with pytest.raises(ValueError):
+        ReadInput(str(f))
+
+
+def test_read_input_reference_example():
+    """The reference's own example input (python/multi_fibers/one_body/
+    data.main, copied as a fixture)."""
+    from fiber3d_amd.fileio import ReadInput
+    r = ReadInput(os.path.join(GOLDEN, "one_body_data.main"))
+    assert r.scheme == "linear_solver_fibers_and_bodies"
+    assert (r.dt, r.n_steps, r.n_save, r.eta, r.E) == (1e-2, 8000, 20, 1.0, 1.0)
+    assert (r.num_points_fibers, r.length_fibers, r.num_points_finite_diff) == (32, 5.0, 11)
+    assert (r.v_shrink, r.rate_catastrophe, r.rate_rescue) == (0.065, 1e-5, 0.014)
+    assert r.solver_tolerance == 1e-12 and r.output_name == "data/run1"
+    assert r.structures_fibers == []         # commented out in the file
+    assert r.structures_body_fibers == [["Structures/body_N_1.vertex",
+                                         "Structures/body_N_1.clones",
+                                         "Structures/body_N_1.fibers",
+                                         "Structures/body_N_1.links"]]
+    assert r.structures_body_fibers_ID == ["body_N_1"]

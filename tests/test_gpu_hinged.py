@@ -1,8 +1,9 @@
 """Fibers whose plus end is hinged, as a fiber that reached the periphery
 (tstep.py:886-888; fiber.py:940-960 for the rows, tstep.py:1167-1168 for the
-flow term), on the GPU against the oracle: right-hand side, operator in FP64
+flow term), and fibers clamped to the cortex (tstep.py:881-883, flow term
+:1162-1166), on the GPU against the oracle: right-hand side, operator in FP64
 and FP32-compensated, preconditioner, and time steps that hold every hinged
-end in place."""
+or clamped end in place."""
 import numpy as np
 import pytest
 import torch
@@ -53,11 +54,13 @@ def _setup():
     return fibers, bodies, attached, hinged, shell, lengths
 
 
-def test_hinged_system_matches_oracle():
-    print("case hinged vs oracle", flush=True)
+@pytest.mark.parametrize("cortex", [False, True])
+def test_hinged_system_matches_oracle(cortex):
+    print(f"case hinged vs oracle cortex={cortex}", flush=True)
     fibers, bodies, attached, hinged, (xs, ns, ws), lengths = _setup()
     force = [np.array([0.3, -0.2, 0.1, 0.05, 0.0, -0.1]), np.array([0.0, 0.2, 0.0, 0.0, 0.1, 0.0])]
-    kw = dict(bodies=bodies, body_force=force, attached=attached, hinged=hinged, lengths=lengths)
+    kw = dict(bodies=bodies, body_force=force, attached=attached, hinged=hinged, lengths=lengths,
+              cortex=cortex)
     gpu = FiberSystem(fibers, Shell(xs, ns, ws), **kw)
     ora = OT.HydroSystem(fibers, (xs, ns, ws), **kw)
     assert rel2(gpu.rhs.cpu().numpy(), ora.rhs) < 1e-11
@@ -102,3 +105,30 @@ def test_hinged_time_steps_hold_the_plus_ends():
         assert t <= h.residual_bound * (1 + 1e-9) + 1e-15
     # the unhinged fiber's plus end is carried by the flow
     assert float(torch.linalg.norm(S.X[S.offs[4] - 1] - free_end)) > 1e-4
+
+
+def test_cortex_time_steps_hold_the_minus_ends():
+    print("case cortex time steps", flush=True)
+    fibers, bodies, attached, hinged, (xs, ns, ws), lengths = _setup()
+    force = [np.array([0.5, 0.0, 0.0, 0.0, 0.0, 0.3]), np.array([0.0, -0.4, 0.0, 0.2, 0.0, 0.0])]
+    ts = TimeStepper(fibers, Shell(xs, ns, ws), mode="fast", tol=1e-8, bodies=bodies,
+                     body_force=force, attached=attached, lengths=lengths, cortex=True,
+                     record=True, predictor=LearnedGuess(use_net=False))
+    S = ts.sys
+    cortex = [1, 2, 3]
+    ends = {k: S.X[S.offs[k]].clone() for k in cortex}
+    tang = {k: S.xs[S.offs[k]].clone() for k in cortex}
+    tip = S.X[S.offs[4] - 1].clone()
+    for _ in range(6):
+        st = ts.step()
+        assert st.residual <= 1e-8
+        for k in cortex:
+            assert torch.linalg.norm(S.X[S.offs[k]] - ends[k]) < 1e-6
+            assert torch.linalg.norm(S.xs[S.offs[k]] - tang[k]) < 1e-5
+    torch.cuda.synchronize()
+    true, drhs = ts.verify()
+    assert max(true) <= 1e-8 and drhs == 0.0
+    for h, t in zip(ts.history, true):
+        assert t <= h.residual_bound * (1 + 1e-9) + 1e-15
+    # the free plus end of a clamped fiber still moves with the flow
+    assert float(torch.linalg.norm(S.X[S.offs[4] - 1] - tip)) > 1e-4

@@ -153,16 +153,19 @@ def test_clamped_fiber_torque_turns_the_assembly():
     assert np.linalg.norm(tip_v - U - np.cross(W, fiber[-1])) < 0.15 * np.linalg.norm(tip_v)
 
 
-def _hinged_step(F, centre=(0.0, 0.0, -0.5), hinged=True, E=10.0, dt=0.02, N=24):
+def _hinged_step(F, centre=(0.0, 0.0, -0.5), hinged=True, E=10.0, dt=0.02, N=24,
+                 cortex=False):
     """A straight fiber along z on [0, 1] with its plus end (z = 1) hinged
-    (tstep.py:886-888) and a sphere of radius 0.3 at centre pushed by F; one
-    implicit step solved to 1e-10 -> (system, new positions, solution)."""
+    (tstep.py:886-888), or with cortex=True its minus end (z = 0) clamped in
+    place (tstep.py:881-883), and a sphere of radius 0.3 at centre pushed by
+    F; one implicit step solved to 1e-10 -> (system, new positions,
+    solution)."""
     body = configs.make_sphere_body(list(centre), 0.3, 300)
     s = np.linspace(0.0, 1.0, N)
     fiber = np.stack([np.zeros(N), np.zeros(N), s], axis=1)
     S = OT.HydroSystem([fiber], None, eta=1.0, dt=dt, E=E, end_force=0.0, bodies=[body],
                        body_force=[np.concatenate([F, np.zeros(3)])],
-                       hinged=[0] if hinged else None)
+                       hinged=[0] if hinged else None, cortex=cortex)
     x, info, _ = OT.gmres(S.matvec, S.rhs, S.precond, tol=1e-10)
     assert info == 0
     Xn = x[S.fib0:S.fib0 + 3 * N].reshape(3, N).T
@@ -200,3 +203,35 @@ def test_hinged_end_holds_the_fiber_against_an_axial_flow():
     assert np.abs(np.linalg.norm(xs, axis=1) - 1.0).max() < 1e-6
     T = x[S.fib0 + 3 * N:S.fib0 + 4 * N]
     assert T[-1] > 0.1
+
+
+def test_cortex_clamped_end_holds_position_and_direction():
+    """A sideways push on a fiber clamped to the cortex at its minus end: the
+    end keeps its place and its tangent while the fiber bends, so the
+    sideways motion grows faster than linearly along it."""
+    F = np.array([2.0, 0.0, 0.0])
+    S, X0, Xn, _ = _hinged_step(F, (0.0, 0.0, 1.5), hinged=False, cortex=True)
+    N = X0.shape[0]
+    v = (Xn - X0) / S.dt
+    assert np.linalg.norm(v[0]) < 1e-7
+    xs = OF.derivatives(S.fm[N], Xn, 1.0)[0]
+    assert np.linalg.norm(xs[0] - [0.0, 0.0, 1.0]) < 1e-8
+    assert v[-1, 0] > 0.0 and np.all(np.diff(v[:, 0]) > 0.0)
+    # bending: the deflection near the clamp is quadratic, not linear
+    assert v[2, 0] < 0.25 * v[-1, 0] * (2.0 / (N - 1))
+
+
+def test_cortex_clamp_holds_the_fiber_against_an_axial_flow():
+    """A push along the fiber away from the cortex: the clamp holds it in
+    tension, |x_s| = 1 to 1e-6, which needs the tangential flow at the
+    clamped end in its tension row (tstep.py:1162-1166)."""
+    F, c = np.array([0.0, 0.0, 3.0]), (0.0, 0.0, -0.5)
+    S, X0, Xn, x = _hinged_step(F, c, hinged=False, cortex=True)
+    N = X0.shape[0]
+    _, _, Xf, _ = _hinged_step(F, c, hinged=False)
+    free = np.abs(Xf - X0).max()
+    assert free > 1e-3 and np.abs(Xn - X0).max() < 1e-2 * free
+    xs = OF.derivatives(S.fm[N], Xn, 1.0)[0]
+    assert np.abs(np.linalg.norm(xs, axis=1) - 1.0).max() < 1e-6
+    T = x[S.fib0 + 3 * N:S.fib0 + 4 * N]
+    assert T[0] > 0.1
