@@ -38,10 +38,12 @@ so the bound never passes and the FP64 product is kept; from a close guess
 remains.
 
 Whenever an inner cycle ends with the FP64 residual instead, the same update
-is set beside it: their difference is the inner operator's error on that
-cycle's correction, ||(A_inner - A) z||, and `measure_inner=True` returns the
-largest such ratio to ||H y|| as `inner_err`, the quantity `inner_rel_err`
-must bound.
+is set beside it.  Their difference, sum_j y_j A_inner P^-1 v_j -
+A P^-1 (sum_j y_j v_j), is exactly what the update misses: the inner
+operator's error on the correction plus the rounding of a low-precision P^-1
+applied to the combination rather than to each basis vector.
+`measure_inner=True` returns the largest such difference relative to ||H y||
+as `inner_err`, the quantity `inner_rel_err` must bound.
 """
 import math
 from dataclasses import dataclass
@@ -166,9 +168,10 @@ def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None, 
                                    inner_cycles + 1, "update", bound / bnorm, measured)
         r = b - matvec(x)
         if r_upd is not None and hyn > 0.0:
-            # r - r_upd = A_inner z and r_old - r = A z for this cycle's
-            # correction z, so their difference is the inner operator's error
-            # on z, measured at no extra cost
+            # r_old - r_upd is the correction's image under the inner
+            # operator as the Arnoldi relation gives it, r_old - r its FP64
+            # image: the difference is the update's whole error, measured at
+            # no extra cost
             e = float(torch.linalg.norm(r - r_upd)) / hyn
             measured = e if measured is None else max(measured, e)
         if op is not matvec:

@@ -35,7 +35,7 @@ In fast mode the closing residual is, by default (certify="update"), the
 guess's FP64 residual updated by the correction's image under the
 FP32-compensated operator, accepted only under a bound that includes that
 operator's error (fiber3d_amd.gmres), an error measured on the system's own
-corrections (calibrate_inner); certify="fp64" recomputes b - A x in FP64
+corrections; certify="fp64" recomputes b - A x in FP64
 instead.  With record=True every step keeps its geometry, right-hand
 side and solution, and verify() re-assembles each recorded step and returns
 the true FP64 residuals.
@@ -106,12 +106,13 @@ class TimeStepper:
         # correction, which the updated-residual certificate adds to its
         # bound.  It depends on the geometry and the state (targets close to
         # a surface, large cancelling forces of a fiber held at both ends),
-        # so it is measured on the corrections of guess-started solves: free
-        # of cost whenever a solve closes with the FP64 residual, and by
-        # calibrate_inner (two products) when no solve has for
-        # 2 lu_every steps.  The bound uses the largest ratio of the last
-        # 4 lu_every steps times a safety factor, never below the floor;
-        # until the first measurement every residual is recomputed in FP64.
+        # so it is measured on the corrections of guess-started solves,
+        # free of cost whenever a solve closes with the FP64 residual.  When
+        # no solve has for 2 lu_every steps, the next solve that iterates is
+        # closed in FP64 to measure it (one FP64 product).  The bound uses
+        # the largest ratio of the last 4 lu_every steps times a safety
+        # factor, never below the floor; until the first measurement every
+        # residual is recomputed in FP64.
         self.inner_rel_err = None
         self.inner_rel_err_floor = 1e-6
         self.inner_rel_err_safety = 10.0
@@ -182,7 +183,9 @@ class TimeStepper:
         a64 = S.matvec(z, precision="f64")
         a32 = S.matvec(z, precision="f32c")
         n = float(torch.linalg.norm(a32))
-        e = float(torch.linalg.norm(a32 - a64)) / n if n > 0 else 0.0
+        if n == 0.0:
+            return None          # no correction, nothing learned
+        e = float(torch.linalg.norm(a32 - a64)) / n
         self._note_inner(e)
         return e
 
@@ -213,9 +216,11 @@ class TimeStepper:
             if self.mode == "fast":
                 fused["fused_inner"] = lambda v, out: G.get("f32c")(v, out=out)
         certify_update = self.mode == "fast" and self.certify == "update"
+        measure_due = (not self._inner_meas
+                       or self.nsteps - self._inner_meas[-1][0] >= 2 * self.lu_every)
         res = gmres(S.matvec, S.rhs, S.precond, tol=self.tol, restart=self.restart,
                     maxiter=self.maxiter, x0=x0, inner=self.inner_operator(),
-                    update_residual=certify_update and self.inner_rel_err is not None,
+                    update_residual=certify_update and not measure_due,
                     inner_rel_err=self.inner_rel_err or 1e-6, target=self.target,
                     measure_inner=certify_update, **fused)
         if certify_update and res.guess_used and res.inner_err is not None:
@@ -259,10 +264,6 @@ class TimeStepper:
         if not accept or dt_new != dt:
             S.dt = dt_new
             S.prepare(refactor=True)
-        if accept and certify_update and res.guess_used and (
-                not self._inner_meas
-                or self.nsteps - self._inner_meas[-1][0] >= 2 * self.lu_every):
-            self.calibrate_inner(sol - x0)
         torch.cuda.synchronize()
         st = StepStats(res.iterations, res.restarts, res.residual, time.perf_counter() - t0,
                        res.guess_used, res.residual_kind, res.residual_bound, accept, dt, err)
