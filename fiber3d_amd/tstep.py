@@ -175,20 +175,6 @@ class TimeStepper:
         worst = max(v for _, v in self._inner_meas)
         self.inner_rel_err = max(self.inner_rel_err_floor, self.inner_rel_err_safety * worst)
 
-    def calibrate_inner(self, z):
-        """Measure ||(A32 - A64) z|| / ||A32 z|| of the assembled operator on
-        a correction z (two products) and fold it into inner_rel_err;
-        returns the ratio."""
-        S = self.sys
-        a64 = S.matvec(z, precision="f64")
-        a32 = S.matvec(z, precision="f32c")
-        n = float(torch.linalg.norm(a32))
-        if n == 0.0:
-            return None          # no correction, nothing learned
-        e = float(torch.linalg.norm(a32 - a64)) / n
-        self._note_inner(e)
-        return e
-
     def inextensibility_error(self):
         """max over all fiber nodes of | |x_s| - 1 | for the assembled
         configuration (tstep.check_error, tstep.py:1874-1891)."""
