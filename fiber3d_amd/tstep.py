@@ -27,9 +27,9 @@ accept/reject scheme (tstep.py:427-581, check_error :1874-1925): after each
 solve the fibers' inextensibility error max |(|x_s| - 1)| of the new
 configuration decides; a step within tol_tstep is accepted (and the next dt
 grows by 1.2, up to dt_max, when the error is below 0.9 tol_tstep), a step
-above it, or one that puts a fiber node on or outside a spherical periphery
-of radius periphery_radius, is rejected and retried with dt / 2 (down to
-dt_min, where it is accepted).
+above it, or one that puts a fiber node on or outside the periphery (a sphere
+of radius periphery_radius, or an ellipsoid of semi-axes periphery_axes), is
+rejected and retried with dt / 2 (down to dt_min, where it is accepted).
 
 In fast mode the closing residual is, by default (certify="update"), the
 guess's FP64 residual updated by the correction's image under the
@@ -67,6 +67,24 @@ class StepStats:
     inextensibility: float = 0.0     # max | |x_s| - 1 | of the new configuration
 
 
+def periphery_collision(X, a, b, c):
+    """True if a fiber node lies on or outside the ellipsoidal periphery of
+    semi-axes (a, b, c) (the fiber-cortex part of tstep.check_collision,
+    tstep.py:1945-1987, with its arithmetic: the node's angles in the
+    axis-scaled frame give a periphery point, and the node collides when it
+    is at least that point's distance from the origin).  X: (n, 3) tensor."""
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    xs, ys, zs = x / a, y / b, z / c
+    r_true = torch.sqrt(x * x + y * y + z * z)
+    r_scaled = torch.sqrt(xs * xs + ys * ys + zs * zs)
+    phi = torch.atan2(ys, xs + 1e-12)
+    theta = torch.arccos(zs / (1e-12 + r_scaled))
+    px = a * torch.sin(theta) * torch.cos(phi)
+    py = b * torch.sin(theta) * torch.sin(phi)
+    pz = c * torch.cos(theta)
+    return bool((r_true >= torch.sqrt(px * px + py * py + pz * pz)).any())
+
+
 def adaptive_decision(err, collision, dt, tol_tstep, dt_min, dt_max):
     """(accept, next dt) of tstep.check_error (tstep.py:1900-1922)."""
     if err <= tol_tstep:
@@ -88,7 +106,7 @@ class TimeStepper:
                  body_force=None, graph=False, certify="update", record=False, lu_every=None,
                  target=None, adaptive=False, tol_tstep=1e-2, dt_min=1e-5, dt_max=1e-2,
                  periphery_radius=None, attached=None, body_orientations=None, hinged=None,
-                 cortex=False):
+                 cortex=False, periphery_axes=None):
         if mode not in ("plain", "fast"):
             raise ValueError("mode is 'plain' or 'fast'")
         if certify not in ("update", "fp64"):
@@ -120,6 +138,7 @@ class TimeStepper:
         self.adaptive = adaptive
         self.tol_tstep, self.dt_min, self.dt_max = tol_tstep, dt_min, dt_max
         self.periphery_radius = periphery_radius
+        self.periphery_axes = None if periphery_axes is None else tuple(map(float, periphery_axes))
         self.time = 0.0          # time of the last accepted solution
         self.rejected = 0
         self.nsteps = 0
@@ -181,11 +200,15 @@ class TimeStepper:
         return float((torch.linalg.norm(self.sys.xs, dim=1) - 1.0).abs().max())
 
     def collision(self):
-        """A fiber node on or outside the spherical periphery (the sphere case
-        of tstep.check_collision, tstep.py:1927-1975)."""
+        """A fiber node on or outside the periphery (the fiber-cortex part of
+        tstep.check_collision, tstep.py:1927-1987); no periphery, no
+        collision."""
+        if self.periphery_axes is not None:
+            return periphery_collision(self.sys.X, *self.periphery_axes)
         if self.periphery_radius is None:
             return False
-        return bool((torch.linalg.norm(self.sys.X, dim=1) >= self.periphery_radius).any())
+        R = self.periphery_radius
+        return periphery_collision(self.sys.X, R, R, R)
 
     def step(self):
         S = self.sys

@@ -400,6 +400,22 @@ class HydroSystem:
         return A
 
 
+def periphery_collision(X, a, b, c):
+    """Fiber-cortex collision of tstep.check_collision (tstep.py:1945-1987):
+    per node, the periphery point at the node's angles in the axis-scaled
+    frame; a node at least as far from the origin as that point collides."""
+    X = np.asarray(X, float)
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    xs, ys, zs = x / a, y / b, z / c
+    r_true = np.sqrt(x ** 2 + y ** 2 + z ** 2)
+    r_scaled = np.sqrt(xs ** 2 + ys ** 2 + zs ** 2)
+    phi = np.arctan2(ys, xs + 1e-12)
+    theta = np.arccos(zs / (1e-12 + r_scaled))
+    p = np.stack([a * np.sin(theta) * np.cos(phi), b * np.sin(theta) * np.sin(phi),
+                  c * np.cos(theta)], axis=1)
+    return bool((r_true >= np.linalg.norm(p, axis=1)).any())
+
+
 def gmres(matvec, b, precond=None, tol=1e-8, restart=150, maxiter=200, x0=None):
     """Right-preconditioned restarted GMRES: solve (A P^-1) y = b, x = P^-1 y.
 

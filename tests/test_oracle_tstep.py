@@ -235,3 +235,27 @@ def test_cortex_clamp_holds_the_fiber_against_an_axial_flow():
     assert np.abs(np.linalg.norm(xs, axis=1) - 1.0).max() < 1e-6
     T = x[S.fib0 + 3 * N:S.fib0 + 4 * N]
     assert T[0] > 0.1
+
+
+def test_periphery_collision_is_the_ellipsoid_inequality():
+    """The reference's angle construction (tstep.py:1945-1987) reduces to
+    (x/a)^2 + (y/b)^2 + (z/c)^2 >= 1: checked on points just inside and
+    just outside, along the axes and off them, for a sphere and an
+    ellipsoid; and the torch version used by the stepper agrees."""
+    import torch
+    from fiber3d_amd.tstep import periphery_collision as gpu_side
+    rng = np.random.default_rng(3)
+    for a, b, c in [(2.0, 2.0, 2.0), (3.0, 2.0, 1.5)]:
+        u = rng.normal(size=(400, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        surf = u / np.sqrt((u[:, 0] / a) ** 2 + (u[:, 1] / b) ** 2 + (u[:, 2] / c) ** 2)[:, None]
+        surf = np.vstack([surf, np.diag([a, b, c]), -np.diag([a, b, c])])
+        for k in range(surf.shape[0]):
+            p = surf[k:k + 1]
+            for f, want in [(1 - 1e-6, False), (1 + 1e-6, True)]:
+                assert OT.periphery_collision(p * f, a, b, c) == want, (p, f)
+                assert gpu_side(torch.from_numpy(p * f), a, b, c) == want
+        inside = surf * 0.5
+        assert not OT.periphery_collision(inside, a, b, c)
+        mixed = np.vstack([inside, surf[:1] * 1.01])
+        assert OT.periphery_collision(mixed, a, b, c)
