@@ -85,6 +85,30 @@ def periphery_collision(X, a, b, c):
     return bool((r_true >= torch.sqrt(px * px + py * py + pz * pz)).any())
 
 
+def fibers_reaching_periphery(fibers, a, b, c, margin=0.05):
+    """Indices of the fibers that reach the periphery of semi-axes (a, b, c),
+    by the reference's test (tstep_utils.py:343-361): some node is on or
+    outside it, or within margin times the periphery point's distance of
+    that point (the point at the node's angles in the axis-scaled frame).
+    The reference hinges such fibers' plus ends; pass the result as hinged.
+    fibers: list of (N_k, 3) host arrays."""
+    out = []
+    for k, x in enumerate(fibers):
+        x = np.asarray(x, dtype=np.float64)
+        xs, ys, zs = x[:, 0] / a, x[:, 1] / b, x[:, 2] / c
+        r_true = np.sqrt((x * x).sum(1))
+        r_scaled = np.sqrt(xs * xs + ys * ys + zs * zs)
+        phi = np.arctan2(ys, xs + 1e-12)
+        theta = np.arccos(zs / (1e-12 + r_scaled))
+        p = np.stack([a * np.sin(theta) * np.cos(phi), b * np.sin(theta) * np.sin(phi),
+                      c * np.cos(theta)], axis=1)
+        d2p = np.sqrt(((x - p) ** 2).sum(1))
+        pr = np.sqrt((p * p).sum(1))
+        if (r_true >= pr).any() or (d2p <= margin * pr).any():
+            out.append(k)
+    return out
+
+
 def adaptive_decision(err, collision, dt, tol_tstep, dt_min, dt_max):
     """(accept, next dt) of tstep.check_error (tstep.py:1900-1922)."""
     if err <= tol_tstep:

@@ -57,3 +57,19 @@ def test_adaptive_decision_rules():
     assert adaptive_decision(0.5e-2, True, 1e-3, tol, lo, hi) == (False, 0.5e-3)
     assert adaptive_decision(0.1e-2, False, 9e-3, tol, lo, hi) == (True, 1e-2)     # capped at dt_max
     assert adaptive_decision(2e-2, False, 1.5e-5, tol, lo, hi) == (True, 1e-5)     # floor: accepted
+
+
+def test_fibers_reaching_periphery():
+    """The reference's periphery-reaching test (tstep_utils.py:343-361) for
+    a sphere and an ellipsoid: a fiber whose plus end lies within 5% of the
+    periphery's distance, or beyond it, is flagged; one stopping short is
+    not."""
+    from fiber3d_amd.tstep import fibers_reaching_periphery
+    s = np.linspace(0.0, 1.0, 16)[:, None]
+    u = np.array([0.6, 0.0, 0.8])
+    for a, b, c in [(2.0, 2.0, 2.0), (3.0, 2.5, 2.0)]:
+        R = 1.0 / np.sqrt((u[0] / a) ** 2 + (u[1] / b) ** 2 + (u[2] / c) ** 2)  # along u
+        fibers = [(R * f - 1.0 + s) * u for f in (0.9, 0.96, 0.999, 1.02)]
+        assert fibers_reaching_periphery(fibers, a, b, c) == [1, 2, 3]
+        # without the margin only a node on or outside the periphery counts
+        assert fibers_reaching_periphery(fibers, a, b, c, margin=0.0) == [3]
