@@ -71,3 +71,22 @@ def test_binding_arity_matches_header():
         name, args = m.group(1), m.group(2).strip()
         n = 0 if args in ("void", "") else len(args.split(","))
         assert n == len(L.SIGNATURES[name][1]), name
+
+
+def test_product_package_does_not_import_the_oracle():
+    """The oracle is the checker only: importing every module of the
+    package in a fresh interpreter loads nothing from oracle/."""
+    import subprocess
+    import sys
+    code = ("import sys, os, pkgutil, importlib, fiber3d_amd\n"
+            "d = fiber3d_amd.__path__[0]\n"
+            "for m in pkgutil.iter_modules(fiber3d_amd.__path__):\n"
+            "    if os.path.exists(os.path.join(d, m.name + '.py')):\n"
+            "        importlib.import_module('fiber3d_amd.' + m.name)\n"
+            "bad = [k for k in sys.modules if k == 'oracle' or k.startswith('oracle.')]\n"
+            "print(bad)\n"
+            "sys.exit(1 if bad else 0)\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
