@@ -359,7 +359,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
-              "the multi-GPU sharding is a design in DESIGN.md, not a measured path.",
+              "sharding over several GPUs is designed but not measured here.",
               file=sys.stderr)
         sys.exit(2)
 
