@@ -20,7 +20,7 @@ tests/golden/make_fiber_golden.py from the reference's own fiber.py /
 fiber_matrices.py (importable here: NumPy/SciPy only) pin the assembly.
 
 The 'chebyshev' kind (Chebyshev-Lobatto nodes, spectral D, first-kind
-collocation grids, Clenshaw-Curtis weights) is this build's reading of the
+collocation grids, Clenshaw-Curtis weights) restates, as read here, the
 older generation (utils/cheb.py, multi_fibers/multi_fiber.py:97); it has no
 live reference counterpart: parity unpinned beyond the polynomial checks.
 """
