@@ -132,3 +132,30 @@ def test_cortex_time_steps_hold_the_minus_ends():
         assert t <= h.residual_bound * (1 + 1e-9) + 1e-15
     # the free plus end of a clamped fiber still moves with the flow
     assert float(torch.linalg.norm(S.X[S.offs[4] - 1] - tip)) > 1e-4
+
+
+def test_hinged_adaptive_rejections_keep_the_ends():
+    """Adaptive steps with a tolerance tight enough to force rejections: a
+    rejected step restores the configuration and re-assembles the hinged and
+    clamped rows at dt / 2, and the hinged ends stay put throughout."""
+    print("case hinged adaptive", flush=True)
+    fibers, bodies, attached, hinged, (xs, ns, ws), lengths = _setup()
+    force = [np.array([0.5, 0.0, 0.0, 0.0, 0.0, 0.3]), np.array([0.0, -0.4, 0.0, 0.2, 0.0, 0.0])]
+    ts = TimeStepper(fibers, Shell(xs, ns, ws), mode="fast", tol=1e-8, bodies=bodies,
+                     body_force=force, attached=attached, hinged=hinged, lengths=lengths,
+                     cortex=True, adaptive=True, dt=4e-3, tol_tstep=1e-9, dt_min=1e-3,
+                     periphery_radius=R_SHELL, record=True,
+                     predictor=LearnedGuess(use_net=False))
+    S = ts.sys
+    ends = {k: S.X[S.offs[k + 1] - 1].clone() for k in hinged}
+    for _ in range(5):
+        st = ts.step()
+        assert st.residual <= 1e-8
+        for k in hinged:
+            assert torch.linalg.norm(S.X[S.offs[k + 1] - 1] - ends[k]) < 1e-6
+    dts = [h.dt for h in ts.history]
+    print("dt per solve", dts, "rejected", ts.rejected, flush=True)
+    assert ts.rejected >= 1 and min(dts) < dts[0]
+    torch.cuda.synchronize()
+    true, drhs = ts.verify()
+    assert max(true) <= 1e-8 and drhs == 0.0
