@@ -13,7 +13,7 @@ from dataclasses import dataclass, field
 import numpy as np
 from scipy.spatial import cKDTree
 
-from .geometry import shape_gallery_sphere
+from .geometry import shape_gallery_ellipsoid, shape_gallery_sphere
 
 
 @dataclass
@@ -115,6 +115,20 @@ def make_periphery(n, radius):
     """Shell nodes, inward normals, equal weights."""
     x, nout = shape_gallery_sphere(n, radius)
     w = np.full(n, 4.0 * np.pi * radius * radius / n)
+    return x, -nout, w
+
+
+def make_periphery_ellipsoid(n, a, b, c):
+    """Ellipsoidal shell of semi-axes (a, b, c): nodes and normals of the
+    reference's ellipsoid gallery, normals turned inward, and weights that
+    carry each Fibonacci cell's equal area 4 pi / N on the unit sphere onto
+    the ellipsoid: w = (4 pi / N) a b c |diag(1/a, 1/b, 1/c) u| for the
+    node's pre-image u (the area factor of the linear map).  The reference
+    computes its weights with its RBF quadrature module instead."""
+    x, nout = shape_gallery_ellipsoid(n, a, b, c)
+    u = x / np.array([a, b, c])
+    w = (4.0 * np.pi / n) * a * b * c * np.sqrt((u[:, 0] / a) ** 2 + (u[:, 1] / b) ** 2
+                                                + (u[:, 2] / c) ** 2)
     return x, -nout, w
 
 

@@ -29,3 +29,17 @@ def shape_gallery_sphere(number_of_nodes, radius):
     normals = 2.0 * nodes
     normals /= np.linalg.norm(normals, axis=1, keepdims=True)
     return nodes, normals
+
+
+def shape_gallery_ellipsoid(number_of_nodes, a, b, c):
+    """Nodes (N,3) and outward unit normals (N,3) of the ellipsoid
+    x^2/a^2 + y^2/b^2 + z^2/c^2 = 1: the Fibonacci lattice of
+    shape_gallery_sphere with its coordinates scaled by (a, b, c), normals
+    along the level-set gradient 2 p / (a^2, b^2, c^2)
+    (shape_gallery.py:34-68)."""
+    u, _ = shape_gallery_sphere(number_of_nodes, 1.0)
+    abc = np.array([a, b, c], dtype=np.float64)
+    nodes = u * abc
+    normals = 2.0 * nodes / abc ** 2
+    normals /= np.linalg.norm(normals, axis=1, keepdims=True)
+    return nodes, normals

@@ -194,3 +194,27 @@ def test_hip_assembly_matches_torch_assembly(N):
     yo = ora.matvec(x)
     assert rel2(y.cpu().numpy(), yo) <= 3.0 * rel2(yt.cpu().numpy(), yo) + 1e-12
     assert rel2(y.cpu().numpy(), yo) < 1e-9
+
+
+def test_ellipsoidal_periphery_matches_oracle():
+    """An ellipsoidal shell (semi-axes 3, 2.5, 2; 900 nodes) around 12
+    fibers: right-hand side, operator in FP64 and FP32-compensated,
+    preconditioner and a GMRES solve against the oracle."""
+    print("case ellipsoidal periphery", flush=True)
+    fibers = configs.make_fibers(configs.FiberSpec(12, 16, 0.6, 1.0), seed=5)
+    shell = configs.make_periphery_ellipsoid(900, 3.0, 2.5, 2.0)
+    L = np.full(len(fibers), 0.6)
+    gpu = FiberSystem(fibers, Shell(*shell), lengths=L)
+    ora = OT.HydroSystem(fibers, shell, lengths=L)
+    assert rel2(gpu.rhs.cpu().numpy(), ora.rhs) < 1e-11
+    x = np.random.default_rng(8).normal(size=ora.size)
+    xt = torch.from_numpy(x).cuda()
+    ref = ora.matvec(x)
+    assert rel2(gpu.matvec(xt).cpu().numpy(), ref) < 1e-10
+    assert rel2(gpu.matvec(xt, precision="f32c").cpu().numpy(), ref) < 1e-6
+    assert rel2(gpu.precond(xt).cpu().numpy(), ora.precond(x)) < 1e-9
+    res = gmres(gpu.matvec, gpu.rhs, gpu.precond, tol=1e-10)
+    torch.cuda.synchronize()
+    assert res.info == 0
+    xo = res.x.cpu().numpy()
+    assert np.linalg.norm(ora.rhs - ora.matvec(xo)) / np.linalg.norm(ora.rhs) < 1.05e-10
