@@ -128,6 +128,12 @@ class FiberMatrices:
         self.N = N = int(N)
         self.kind = kind
         if kind == "reference":
+            # the k-th derivative uses nfd + k of the N nodes
+            # (num_points_finite_diff, fiber_matrices.py:24-31, :58-66); the
+            # reference's Fornberg loop fails outright outside this range
+            if not 1 <= nfd <= N - 4:
+                raise ValueError("num_points_finite_diff must be in [1, N - 4] = [1, %d], got %r"
+                                 % (N - 4, nfd))
             self.alpha = np.linspace(-1.0, 1.0, N)
             self.alpha_roots = -1.0 + (2.0 * np.arange(N - 4) + 1.0) / (N - 4)
             self.alpha_tension = -1.0 + (2.0 * np.arange(N - 2) + 1.0) / (N - 2)

@@ -107,3 +107,15 @@ def test_chebyshev_variant_spectral():
     assert abs(fm.weights0.sum() - 2.0) < 1e-13
     # first-kind collocation grids interpolate a polynomial exactly
     assert rel2(fm.P_X @ a**5, fm.alpha_roots**5) < 1e-12
+
+
+def test_stencil_width_out_of_range_is_rejected():
+    """num_points_finite_diff outside [1, N - 4] leaves the k-th derivative
+    without its nfd + k nodes: rejected with a clear error (the reference's
+    Fornberg loop fails with an IndexError there)."""
+    import pytest
+    from fiber3d_amd.fiber_matrices import FiberMatrices
+    FiberMatrices(16, 12)
+    for bad in (0, 13):
+        with pytest.raises(ValueError):
+            FiberMatrices(16, bad)
