@@ -46,8 +46,8 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from .fileio import (write_body_velocities_frame, write_clones_frame, write_fibers_frame,
-                     write_resume_clones, write_resume_fibers)
+from .fileio import (fibers_from_file, write_body_velocities_frame, write_clones_frame,
+                     write_fibers_frame, write_resume_clones, write_resume_fibers)
 from .gmres import gmres
 from .graphs import ArnoldiGraphs
 from .system import FiberSystem
@@ -367,3 +367,28 @@ class TimeStepper:
         S.dt = now[2]
         S.prepare()
         return out, drhs
+
+
+def stepper_from_input(read, shell, **kw):
+    """A TimeStepper set up from a ReadInput (read_input/read_input.py) and a
+    Shell: the fibers of every structure_fibers file (each fiber's E and
+    length from its header, else E and length_fibers), dt, eta, epsilon,
+    nfd = num_points_finite_diff (4 when unset) and the GMRES tolerance
+    solver_tolerance.  Keyword arguments are passed on and take precedence.
+    One elastic modulus serves all fibers, so differing header values are
+    rejected."""
+    fibers, Es, Ls = [], [], []
+    for name in read.structures_fibers:
+        x, E, L = fibers_from_file(name, E=read.E, length=read.length_fibers)
+        fibers += x
+        Es += E
+        Ls += L
+    if not fibers:
+        raise ValueError("the input file names no structure_fibers file")
+    if len(set(Es)) != 1:
+        raise ValueError("fibers with different E are not supported: %s" % sorted(set(Es)))
+    args = dict(dt=read.dt, eta=read.eta, E=Es[0], epsilon=read.epsilon,
+                nfd=read.num_points_finite_diff or 4, tol=read.solver_tolerance,
+                lengths=np.asarray(Ls, dtype=np.float64))
+    args.update(kw)
+    return TimeStepper(fibers, shell, **args)

@@ -84,3 +84,25 @@ def test_body_orientations_track_rotation_and_write(tmp_path):
     assert rows.shape == (5, 7) and abs(rows[1, 0] - 0.5) < 1e-12
     vel = np.loadtxt(io.StringIO(vbuf.getvalue().decode()))
     assert np.array_equal(vel[1:], ts.body_velocity.cpu().numpy())
+
+
+def test_stepper_from_input_file(tmp_path):
+    """stepper_from_input: an input file in the reference's format naming
+    the reference's example fibers sets up the stepper (dt, eta, tolerance,
+    fibers, E and lengths from the headers), which then steps to its
+    tolerance."""
+    print("case stepper from input", flush=True)
+    from fiber3d_amd.fileio import ReadInput
+    from fiber3d_amd.tstep import stepper_from_input
+    f = tmp_path / "data.main"
+    f.write_text("dt 1e-3\neta 1.0\nsolver_tolerance 1e-10\nE 0.5\n"
+                 "structure_fibers %s\n" % os.path.join(GOLDEN, "two.fibers"))
+    r = ReadInput(str(f))
+    xsh, nsh, wsh = configs.make_periphery(800, 8.0)
+    ts = stepper_from_input(r, Shell(xsh, nsh, wsh), mode="plain", discretisation="chebyshev")
+    xs, Es, Ls = fibers_from_file(os.path.join(GOLDEN, "two.fibers"), E=0.5)
+    assert ts.sys.dt == 1e-3 and ts.tol == 1e-10 and ts.sys.E == Es[0]
+    np.testing.assert_array_equal(ts.sys.lengths_h, Ls)
+    np.testing.assert_array_equal(ts.sys.X.cpu().numpy(), np.concatenate(xs))
+    for _ in range(2):
+        assert ts.step().residual <= 1e-10
