@@ -20,6 +20,9 @@ Output is byte-for-byte what the reference writes for the same values.
 The simulation input file (options, structures) follows
 read_input/read_input.py:9-191 (ReadInput).
 
+Body vertex and link files follow read_input/read_vertex_file.py and
+read_input/read_links_file.py.
+
 Rigid bodies (clones) follow read_input/read_clones_file.py:9-48 and the
 body part of tstep.write_to_file (tstep.py:2078-2123): a count, then one
 "x y z s p1 p2 p3" line per body (location and orientation quaternion,
@@ -237,6 +240,37 @@ def write_resume_fibers(name_file, positions, E, lengths, growing=None):
             x = np.asarray(positions[k], dtype=np.float64)
             f.write(("%s %s %s %s \n" % (x.shape[0], E[k], lengths[k], growing[k])).encode("utf-8"))
             _rows(f, x)
+
+
+def _data_lines(name_file):
+    """Non-empty lines of a reference text file, '#' comments removed."""
+    with open(name_file, "r") as f:
+        for line in f:
+            line = line.split("#", 1)[0].strip()
+            if line:
+                yield line.split()
+
+
+def read_vertex_file(name_file):
+    """Body vertex file (read_input/read_vertex_file.py:7-33): a count line,
+    then "x y z" per node -> (Nblobs, 3) array (the count is read, not
+    enforced, as in the reference)."""
+    rows = list(_data_lines(name_file))
+    return np.array([[float(v) for v in r[:3]] for r in rows[1:]])
+
+
+def read_links_file(name_file):
+    """Links file (read_input/read_links_file.py:7-48): a count line, then
+    "k k_angle x y z ax ay az" per link -> (spring constants (n, 1),
+    angular spring constants (n, 1), locations (n, 3), unit axes (n, 3))."""
+    rows = list(_data_lines(name_file))[1:]
+    k = np.array([[float(r[0])] for r in rows])
+    ka = np.array([[float(r[1])] for r in rows])
+    loc = np.array([[float(v) for v in r[2:5]] for r in rows])
+    ax = np.array([[float(v) for v in r[5:8]] for r in rows])
+    if len(rows):
+        ax = ax / np.linalg.norm(ax, axis=1, keepdims=True)
+    return k, ka, loc, ax
 
 
 def read_clones_file(name_file):

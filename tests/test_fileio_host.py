@@ -190,3 +190,21 @@ def test_read_input_reference_example():
                                          "Structures/body_N_1.fibers",
                                          "Structures/body_N_1.links"]]
     assert r.structures_body_fibers_ID == ["body_N_1"]
+
+
+def test_vertex_and_links_files(tmp_path):
+    """Body vertex and link files in the reference's formats
+    (read_input/read_vertex_file.py, read_links_file.py): comments and blank
+    lines skipped, link axes normalised."""
+    from fiber3d_amd.fileio import read_links_file, read_vertex_file
+    v = tmp_path / "body.vertex"
+    v.write_text("# nodes\n3\n0 0 1\n\n1.5 -2 0.25  # a comment\n0 1e-3 -1\n")
+    np.testing.assert_array_equal(read_vertex_file(str(v)),
+                                  [[0, 0, 1], [1.5, -2, 0.25], [0, 1e-3, -1]])
+    l = tmp_path / "body.links"
+    l.write_text("2\n10 5 0 0 0.5 0 0 2\n# second\n1 0.5 0.3 0 0.4 3 4 0\n")
+    k, ka, loc, ax = read_links_file(str(l))
+    np.testing.assert_array_equal(k, [[10.0], [1.0]])
+    np.testing.assert_array_equal(ka, [[5.0], [0.5]])
+    np.testing.assert_array_equal(loc, [[0, 0, 0.5], [0.3, 0, 0.4]])
+    np.testing.assert_allclose(ax, [[0, 0, 1], [0.6, 0.8, 0]], rtol=0, atol=1e-15)
