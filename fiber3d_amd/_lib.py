@@ -19,6 +19,7 @@ OSEEN, ROTLET, STRESSLET = 0, 1, 2
 F64, F32C = 0, 1
 ACCUMULATE = 1
 CLAMP_NEAR = 2
+EXCLUDE_SELF = 4
 
 _c_p = ctypes.c_void_p
 _i64 = ctypes.c_int64

@@ -201,9 +201,8 @@ pack_kernel(int ns, int ns_pad, const double* __restrict__ pos, const double* __
         r[6] = 0; r[7] = 0;
       }
     } else {
-      // Zero-density padding contributes exactly zero: its r is finite because
-      // the near-pair regularisation keeps r >= reg (a target sitting at the
-      // padding position is regularised like any coincident pair).
+      // Zero-density padding; the pair kernel masks sources j >= ns, so a
+      // target at the padding position never meets an unregularised r = 0.
 #pragma unroll
       for (int c = 0; c < R; ++c) r[c] = 0;
     }
@@ -271,9 +270,12 @@ __device__ __forceinline__ void pair(const T* r, TCoord<T> tx, TCoord<T> ty, TCo
 }
 
 // One 64-source tile; returns (wave-uniformly) whether any pair was near when
-// MODE == MODE_VOTE.
+// MODE == MODE_VOTE.  MASK: drop the pairs of excluded ranges and of the
+// zero-padding sources j >= ns (a padding record may sit on a target, where
+// an unregularised 1/r would be infinite).
 template <typename T, int KIND, int K, int MODE, bool MASK>
-__device__ __forceinline__ bool tile(const T* __restrict__ rec, int j0, const TCoord<T> (&tx)[K],
+__device__ __forceinline__ bool tile(const T* __restrict__ rec, int j0, int ns,
+                                     const TCoord<T> (&tx)[K],
                                      const TCoord<T> (&ty)[K], const TCoord<T> (&tz)[K],
                                      const int (&lo)[K], const int (&hi)[K], T reg2, T eps2,
                                      T rmax, T (&ax)[K], T (&ay)[K], T (&az)[K]) {
@@ -295,7 +297,8 @@ __device__ __forceinline__ bool tile(const T* __restrict__ rec, int j0, const TC
       const int j = j0 + jj + u;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const bool excl = MASK && ((unsigned)(j - lo[k]) < (unsigned)(hi[k] - lo[k]));
+        const bool excl =
+            MASK && (((unsigned)(j - lo[k]) < (unsigned)(hi[k] - lo[k])) || j >= ns);
         pair<T, KIND, MODE, MASK>(s[u], tx[k], ty[k], tz[k], reg2, eps2, rmax, excl, ax[k],
                                   ay[k], az[k], vote);
       }
@@ -308,9 +311,10 @@ __device__ __forceinline__ bool tile(const T* __restrict__ rec, int j0, const TC
 // the negated Kahan compensation.
 template <typename T, int KIND, int K, bool CLAMP>
 __global__ void __launch_bounds__(kBlock)
-pairsum_kernel(const T* __restrict__ rec, int chunk, int ns_pad, const double* __restrict__ tpos,
-               long t_sp, long t_sc, int nt, const int* __restrict__ excl_lo,
-               const int* __restrict__ excl_hi, double reg, double eps,
+pairsum_kernel(const T* __restrict__ rec, int chunk, int ns, int ns_pad,
+               const double* __restrict__ tpos, long t_sp, long t_sc, int nt,
+               const int* __restrict__ excl_lo, const int* __restrict__ excl_hi, int self_excl,
+               double reg, double eps,
                const Geo* __restrict__ geo, T* __restrict__ part_hi, T* __restrict__ part_lo) {
   constexpr bool kQuant = std::is_same<T, float>::value;
   // regularisation constants in the units of the coordinates used
@@ -338,9 +342,9 @@ pairsum_kernel(const T* __restrict__ rec, int chunk, int ns_pad, const double* _
     }
     lo[k] = 0;
     hi[k] = 0;
-    if (excl_lo != nullptr && i < nt) {
-      lo[k] = excl_lo[i];
-      hi[k] = excl_hi[i];
+    if ((excl_lo != nullptr || self_excl) && i < nt) {
+      lo[k] = self_excl ? i : excl_lo[i];
+      hi[k] = self_excl ? i + 1 : excl_hi[i];
       if (hi[k] > lo[k]) {
         wlo = min(wlo, lo[k]);
         whi = max(whi, hi[k]);
@@ -361,22 +365,22 @@ pairsum_kernel(const T* __restrict__ rec, int chunk, int ns_pad, const double* _
     T ax[K], ay[K], az[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = 0;
-    if (j0 < whi && j0 + kTile > wlo) {
+    if ((j0 < whi && j0 + kTile > wlo) || j0 + kTile > ns) {
       if (CLAMP)
-        tile<T, KIND, K, MODE_CLAMP, true>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax,
+        tile<T, KIND, K, MODE_CLAMP, true>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax,
                                            ay, az);
       else
-        tile<T, KIND, K, MODE_EXACT, true>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax,
+        tile<T, KIND, K, MODE_EXACT, true>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax,
                                            ay, az);
     } else if (CLAMP) {
-      tile<T, KIND, K, MODE_CLAMP, false>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax, ay,
+      tile<T, KIND, K, MODE_CLAMP, false>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax, ay,
                                           az);
-    } else if (tile<T, KIND, K, MODE_VOTE, false>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax,
+    } else if (tile<T, KIND, K, MODE_VOTE, false>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax,
                                                   ax, ay, az)) {
       // some lane met a pair with r < eps: redo the tile with the regularised form
 #pragma unroll
       for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = 0;
-      tile<T, KIND, K, MODE_EXACT, false>(rec, j0, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax, ay,
+      tile<T, KIND, K, MODE_EXACT, false>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax, ay,
                                           az);
     }
 #pragma unroll
@@ -534,11 +538,14 @@ static size_t part_bytes(int precision, int P, int64_t nt) {
   return align_up((size_t)P * 3 * (size_t)(nt > 0 ? nt : 1) * per, 256);
 }
 
-static size_t pairsum_ws(int kernel, int precision, int64_t ns, int64_t nt) {
-  const Plan pl = make_plan(kernel, precision, ns, nt);
+static size_t plan_ws(int kernel, int precision, const Plan& pl, int64_t nt) {
   return rec_bytes(kernel, precision, pl.ns_pad) + part_bytes(precision, pl.P, nt) +
          align_up(kPackBlocks * sizeof(double), 256) + align_up(sizeof(Geo), 256) +
          align_up(kBoxBlocks * 6 * sizeof(double), 256);
+}
+
+static size_t pairsum_ws(int kernel, int precision, int64_t ns, int64_t nt) {
+  return plan_ws(kernel, precision, make_plan(kernel, precision, ns, nt), nt);
 }
 
 template <typename T, int KIND>
@@ -575,12 +582,13 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
 
   const dim3 grid((unsigned)ceil_div(nt, (int64_t)kBlock * pl.K), (unsigned)pl.P);
   const bool clamp = kQuant && (flags & F3D_CLAMP_NEAR);
+  const int self_excl = (flags & F3D_EXCLUDE_SELF) ? 1 : 0;
   const bool timed = g_timer.on && g_timer.n < kTimerSlots;
   if (timed) F3D_HIP_CHECK(hipEventRecord(g_timer.ev[2 * g_timer.n], st));
 #define F3D_PS(KK, CL)                                                                        \
   F3D_LAUNCH((pairsum_kernel<T, KIND, KK, CL>), grid, dim3(kBlock), 0, st, rec, pl.chunk,     \
-             pl.ns_pad, tpos, (long)t_sp, (long)t_sc, (int)nt, lo, hi, reg, eps,              \
-             (const Geo*)geo, part_hi, part_lo)
+             (int)ns, pl.ns_pad, tpos, (long)t_sp, (long)t_sc, (int)nt, lo, hi, self_excl,    \
+             reg, eps, (const Geo*)geo, part_hi, part_lo)
   if (clamp) {
     if (pl.K == 1) F3D_PS(1, true);
     else if (pl.K == 2) F3D_PS(2, true);
@@ -625,6 +633,10 @@ static int pairsum_impl(int kernel, int precision, uint32_t flags, int64_t ns, c
               "pairsum: ns=%lld nt=%lld exceed 2^30", (long long)ns, (long long)nt);
   F3D_REQUIRE((lo == nullptr) == (hi == nullptr), F3D_EINVAL,
               "pairsum: excl_lo and excl_hi must both be set or both NULL");
+  F3D_REQUIRE(!((flags & F3D_EXCLUDE_SELF) && lo != nullptr), F3D_EINVAL,
+              "pairsum: F3D_EXCLUDE_SELF and exclusion ranges are exclusive");
+  F3D_REQUIRE((flags & ~(F3D_ACCUMULATE | F3D_CLAMP_NEAR | F3D_EXCLUDE_SELF)) == 0u, F3D_EINVAL,
+              "pairsum: unknown flags 0x%x", flags);
   F3D_REQUIRE(eta > 0.0, F3D_EINVAL, "pairsum: eta must be positive");
   F3D_REQUIRE(eps >= 0.0 && reg >= 0.0, F3D_EINVAL, "pairsum: eps and reg must be >= 0");
   if (nt == 0) return F3D_OK;
@@ -641,10 +653,10 @@ static int pairsum_impl(int kernel, int precision, uint32_t flags, int64_t ns, c
   F3D_REQUIRE(pos != nullptr && den != nullptr, F3D_EINVAL, "pairsum: NULL source arrays");
   F3D_REQUIRE(kernel != F3D_STRESSLET || nrm != nullptr, F3D_EINVAL,
               "pairsum: stresslet needs normals");
-  const size_t need = pairsum_ws(kernel, precision, ns, nt);
+  const Plan pl = make_plan(kernel, precision, ns, nt);
+  const size_t need = plan_ws(kernel, precision, pl, nt);
   F3D_REQUIRE(ws != nullptr && ws_bytes >= need, F3D_EWORKSPACE,
               "pairsum: workspace %zu bytes < required %zu", ws_bytes, need);
-  const Plan pl = make_plan(kernel, precision, ns, nt);
   char* wsc = reinterpret_cast<char*>(ws);
 #define F3D_ARGS pl, flags, ns, pos, nrm, den, w, s_sp, s_sc, nt, tpos, t_sp, t_sc, lo, hi, eta, \
                  reg, eps, out, o_sp, o_sc, wsc, epi, st
@@ -737,7 +749,7 @@ extern "C" int f3d_periphery_apply(int precision, int64_t n, const double* pos,
   F3D_REQUIRE(n > 0, F3D_EINVAL, "periphery_apply: n must be positive");
   F3D_REQUIRE(nrm && w && ex && ey && ez && q, F3D_EINVAL, "periphery_apply: NULL argument");
   ShellEpi epi{nrm, w, ex, ey, ez, q, add, nullptr};
-  return pairsum_impl(F3D_STRESSLET, precision, 0u, n, pos, nrm, q, nullptr, sp, sc, n, pos, sp,
+  return pairsum_impl(F3D_STRESSLET, precision, F3D_EXCLUDE_SELF, n, pos, nrm, q, nullptr, sp, sc, n, pos, sp,
                       sc, nullptr, nullptr, eta, reg, eps, out, sp, sc, ws, ws_bytes, &epi,
                       stream);
 }

@@ -105,13 +105,14 @@ def _as_common(ts):
 
 def pairsum(kernel, src_pos, src_den, trg_pos, src_nrm=None, src_w=None, excl=None,
             eta=1.0, reg=5e-3, eps=1e-5, precision="f64", out=None, accumulate=False,
-            clamp_near=False):
+            clamp_near=False, exclude_self=False):
     """Device pair sum (libf3d f3d_pairsum) on torch CUDA tensors.
 
     kernel: 'oseen' | 'rotlet' | 'stresslet'.  src_pos, src_den, src_nrm,
     trg_pos, out: (N,3) float64 CUDA tensors with any strides (an SoA buffer
     (3,N) is passed as its transpose).  src_w: (Ns,) float64 weights.
     excl: (lo, hi) int32 (Nt,) tensors of excluded source ranges.
+    exclude_self: drop the pair (source i, target i) (self-sums, j != i).
     Returns out (allocated AoS if None).
     """
     k = {"oseen": L.OSEEN, "rotlet": L.ROTLET, "stresslet": L.STRESSLET}[kernel]
@@ -133,7 +134,8 @@ def pairsum(kernel, src_pos, src_den, trg_pos, src_nrm=None, src_w=None, excl=No
     prec = _PREC[precision]
     need = lib.f3d_pairsum_workspace_size(k, prec, ns, nt)
     ws = workspace(need)
-    flags = (L.ACCUMULATE if accumulate else 0) | (L.CLAMP_NEAR if clamp_near else 0)
+    flags = ((L.ACCUMULATE if accumulate else 0) | (L.CLAMP_NEAR if clamp_near else 0)
+             | (L.EXCLUDE_SELF if exclude_self else 0))
     rc = lib.f3d_pairsum(k, prec, flags, ns, L.ptr(sp_), L.ptr(sn_), L.ptr(sd_), L.ptr(src_w),
                          s_sp, s_sc, nt, L.ptr(trg_pos), t_sp, t_sc, L.ptr(lo), L.ptr(hi),
                          float(eta), float(reg), float(eps), L.ptr(out), o_sp, o_sc,
@@ -188,13 +190,13 @@ def _to_dev(a, n=None):
     return torch.from_numpy(a).to(device())
 
 
-def _np_pairsum(kernel, r_source, r_target, density, normal, eta, reg, eps, excl=None):
+def _np_pairsum(kernel, r_source, r_target, density, normal, eta, reg, eps, self_sum=False):
     s = _to_dev(r_source)
-    t = _to_dev(r_target)
+    t = s if self_sum else _to_dev(r_target)
     d = _to_dev(density, s.shape[0])
     nm = _to_dev(normal, s.shape[0]) if normal is not None else None
-    u = pairsum(kernel, s, d, t, src_nrm=nm, excl=excl, eta=eta, reg=reg, eps=eps,
-                precision=_precision)
+    u = pairsum(kernel, s, d, t, src_nrm=nm, eta=eta, reg=reg, eps=eps,
+                precision=_precision, exclude_self=self_sum)
     return u.cpu().numpy().ravel()
 
 
@@ -220,10 +222,10 @@ def stresslet_kernel_source_target_numba(r_source, r_target, normal, density, et
 def stresslet_kernel_times_normal_times_density_numba(r_vectors, normal, density, eta=1.0,
                                                       reg=5e-3, epsilon_distance=1e-5):
     """Drop-in for kernels.stresslet_kernel_times_normal_times_density_numba
-    (kernels/kernels.py:1123-1183): the self-sum over j != i.  The i == j pair
-    has d = 0 exactly and contributes exactly zero, as the reference's skip."""
+    (kernels/kernels.py:1123-1183): the self-sum over j != i (the i == j pair
+    is dropped by F3D_EXCLUDE_SELF, as the reference's loop skips it)."""
     return _np_pairsum("stresslet", r_vectors, r_vectors, density, normal, eta, reg,
-                       epsilon_distance)
+                       epsilon_distance, self_sum=True)
 
 
 def complementary_kernel_times_density_numba(r_vectors, normal, density):

@@ -97,7 +97,7 @@ class Shell:
             dens = torch.zeros_like(self.pos)
             dens[:, c] = self.w
             self.e.append(pairsum("stresslet", self.pos, dens, self.pos, src_nrm=self.nrm,
-                                  eta=eta, reg=reg, eps=eps))
+                                  eta=eta, reg=reg, eps=eps, exclude_self=True))
         if pc == "auto":
             pc = "dense" if self.n <= self.DENSE_MAX else "block"
         self.pc = pc
@@ -215,7 +215,7 @@ class Bodies:
             for k in range(self.count):
                 s = slice(k * nb, (k + 1) * nb)
                 pairsum("stresslet", self.pos[s], dens[s], self.pos[s], src_nrm=self.nrm[s],
-                        eta=self.eta, reg=self.reg, eps=self.eps, out=e[s])
+                        eta=self.eta, reg=self.reg, eps=self.eps, out=e[s], exclude_self=True)
             e_new.append(e)
         if hasattr(self, "e"):
             for c in range(3):
@@ -850,9 +850,9 @@ class FiberSystem:
         if B is not None:
             lam = B.density_view(x, nper3)
             vb = B.vb
-            # body double layer on fibers and bodies (the j = i term is exactly 0)
+            # body double layer on fibers and bodies (the j = i pair dropped)
             pairsum("stresslet", B.pos, lam, self.X, src_nrm=B.nrm, out=vsoa, accumulate=True, **kw)
-            pairsum("stresslet", B.pos, lam, B.pos, src_nrm=B.nrm, out=vb, **kw)
+            pairsum("stresslet", B.pos, lam, B.pos, src_nrm=B.nrm, out=vb, exclude_self=True, **kw)
             pairsum("oseen", self.X, fsoa, B.pos, src_w=self.weights, out=vb, accumulate=True, **kw)
             if ft is not None:
                 link_flow(self.X, vsoa)

@@ -48,6 +48,10 @@ extern "C" {
                                  r < eps, clamp 1/r at 1/eps (one instruction
                                  less per pair; identical results whenever no
                                  non-excluded pair is closer than eps)        */
+#define F3D_EXCLUDE_SELF 4u   /* drop the pair (source i, target i): for a
+                                 self-sum with the points as both sources
+                                 and targets (the reference's j != i loop);
+                                 not combinable with excl_lo/excl_hi          */
 
 /* Returns a short description of the last error on the calling thread. */
 const char* f3d_last_error(void);
@@ -87,8 +91,8 @@ size_t f3d_pairsum_workspace_size(int kernel, int precision, int64_t ns, int64_t
  *   kernels.stresslet_kernel_source_target_numba  (python/kernels/kernels.py:1186-1217)
  *   kernels.stresslet_kernel_times_normal_times_density_numba
  *                                                 (python/kernels/kernels.py:1123-1183;
- *      call with the same points as sources and targets: the i==j term is
- *      exactly zero for the stresslet, as in the reference's j != i loop)
+ *      call with the same points as sources and targets and F3D_EXCLUDE_SELF,
+ *      the reference's j != i loop)
  * and, with src_w != NULL, the weight multiply of tstep_utils.flow_fibers
  *   (python/tstep/tstep_utils.py:653-700).
  *

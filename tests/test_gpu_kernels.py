@@ -145,3 +145,44 @@ def test_large_oseen_f32c_against_f64_and_oracle_rows():
     u2 = FK.pairsum("oseen", X, 2.0 * F - F, X, excl=excl, precision="f64")
     torch.cuda.synchronize()
     assert rel2(u2.cpu().numpy(), u64.cpu().numpy()) < 1e-13
+
+
+@pytest.mark.parametrize("kernel", ["oseen", "rotlet", "stresslet"])
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_padding_sources_are_masked_without_regularisation(kernel, precision):
+    """With eps = reg = 0 nothing is regularised.  100 sources leave 28
+    zero-density padding records in the last 64-source tile, placed at the
+    origin (FP64) or at the bounding box's minimum corner (FP32C, quantised
+    0); a target sits exactly there.  The reference's sum over the real
+    sources is finite at that target, and so must ours be."""
+    print(f"case padding {kernel} {precision}", flush=True)
+    rng = np.random.default_rng(21)
+    s = rng.uniform(0.1, 1.0, size=(100, 3))
+    t = np.vstack([np.zeros((1, 3)), rng.uniform(0.1, 1.0, size=(40, 3))])
+    d = rng.normal(size=(100, 3))
+    n = rng.normal(size=(100, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    ref = ORACLE[kernel](s, t, d, n, reg=0.0, epsilon_distance=0.0)
+    assert np.isfinite(ref).all()
+    got = FK.pairsum(kernel, _dev(s), _dev(d), _dev(t), src_nrm=_dev(n), reg=0.0, eps=0.0,
+                     precision=precision).cpu().numpy().ravel()
+    assert np.isfinite(got).all()
+    assert rel2(got, ref) < TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_self_sum_drops_the_diagonal_without_regularisation(precision):
+    """The stresslet self-sum (j != i) with eps = reg = 0: the i == j pair is
+    dropped by the self-pair flag rather than relying on a regularised
+    r = 0 term."""
+    rng = np.random.default_rng(22)
+    x = rng.uniform(-1, 1, size=(300, 3))
+    n = rng.normal(size=(300, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    q = rng.normal(size=(300, 3))
+    ref = K.stresslet_kernel_times_normal_times_density(x, n, q, reg=0.0, epsilon_distance=0.0)
+    assert np.isfinite(ref).all()
+    X = _dev(x)
+    got = FK.pairsum("stresslet", X, _dev(q), X, src_nrm=_dev(n), reg=0.0, eps=0.0,
+                     precision=precision, exclude_self=True).cpu().numpy().ravel()
+    assert rel2(got, ref) < TOL[precision]
