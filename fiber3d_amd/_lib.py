@@ -45,6 +45,13 @@ SIGNATURES = {
         _f64, _f64, _f64,
         _c_p, _i64, _i64,
         _c_p, _sz, _c_p]),
+    "f3d_pairsum_noalloc": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, _u32,
+        _i64, _c_p, _c_p, _c_p, _c_p, _i64, _i64,
+        _i64, _c_p, _i64, _i64,
+        _c_p, _c_p,
+        _f64, _f64, _f64,
+        _c_p, _i64, _i64, _c_p]),
     "f3d_periphery_workspace_size": (_sz, [ctypes.c_int, _i64]),
     "f3d_periphery_apply": (ctypes.c_int, [
         ctypes.c_int, _i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,

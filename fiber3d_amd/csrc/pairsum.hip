@@ -36,6 +36,7 @@
 #include <stdarg.h>
 #include <stdlib.h>
 
+#include <mutex>
 #include <type_traits>
 
 namespace f3d {
@@ -548,6 +549,27 @@ static size_t pairsum_ws(int kernel, int precision, int64_t ns, int64_t nt) {
   return plan_ws(kernel, precision, make_plan(kernel, precision, ns, nt), nt);
 }
 
+// The default plan with fewer source chunks (fewer partial sums) where its
+// workspace exceeds cap bytes; P = 0 when even one chunk does not fit.
+static Plan make_plan_within(int kernel, int precision, int64_t ns, int64_t nt, size_t cap) {
+  Plan pl = make_plan(kernel, precision, ns, nt);
+  if (plan_ws(kernel, precision, pl, nt) <= cap) return pl;
+  Plan one = pl;
+  one.P = 1;
+  const size_t fixed = plan_ws(kernel, precision, one, nt) - part_bytes(precision, 1, nt);
+  const size_t per = 3 * (size_t)(nt > 0 ? nt : 1) * 8;
+  int64_t pmax = cap > fixed + 256 ? (int64_t)((cap - fixed - 256) / per) : 0;
+  if (pmax < 1) {
+    pl.P = 0;
+    return pl;
+  }
+  const int64_t tiles = pl.ns_pad / kTile;
+  const int64_t tiles_per = ceil_div(tiles, pmax < tiles ? pmax : tiles);
+  pl.chunk = (int)(tiles_per * kTile);
+  pl.P = (int)ceil_div(pl.ns_pad, pl.chunk);
+  return pl;
+}
+
 template <typename T, int KIND>
 static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const double* pos,
                           const double* nrm, const double* den, const double* w, int64_t s_sp,
@@ -618,12 +640,15 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
   return F3D_OK;
 }
 
+// fit_ws: shrink the source split until the workspace fits ws_bytes (the
+// allocation-free entry point, whose scratch has a fixed size) instead of
+// requiring the default plan's size.
 static int pairsum_impl(int kernel, int precision, uint32_t flags, int64_t ns, const double* pos,
                         const double* nrm, const double* den, const double* w, int64_t s_sp,
                         int64_t s_sc, int64_t nt, const double* tpos, int64_t t_sp, int64_t t_sc,
                         const int32_t* lo, const int32_t* hi, double eta, double reg, double eps,
                         double* out, int64_t o_sp, int64_t o_sc, void* ws, size_t ws_bytes,
-                        const ShellEpi* epi, void* stream) {
+                        const ShellEpi* epi, void* stream, bool fit_ws = false) {
   F3D_REQUIRE(kernel == F3D_OSEEN || kernel == F3D_ROTLET || kernel == F3D_STRESSLET, F3D_EINVAL,
               "pairsum: unknown kernel %d", kernel);
   F3D_REQUIRE(precision == F3D_F64 || precision == F3D_F32C, F3D_EINVAL,
@@ -653,7 +678,15 @@ static int pairsum_impl(int kernel, int precision, uint32_t flags, int64_t ns, c
   F3D_REQUIRE(pos != nullptr && den != nullptr, F3D_EINVAL, "pairsum: NULL source arrays");
   F3D_REQUIRE(kernel != F3D_STRESSLET || nrm != nullptr, F3D_EINVAL,
               "pairsum: stresslet needs normals");
-  const Plan pl = make_plan(kernel, precision, ns, nt);
+  Plan pl;
+  if (fit_ws) {
+    pl = make_plan_within(kernel, precision, ns, nt, ws_bytes);
+    F3D_REQUIRE(pl.P > 0 && plan_ws(kernel, precision, pl, nt) <= ws_bytes, F3D_EWORKSPACE,
+                "pairsum: ns=%lld nt=%lld need more than the %zu bytes of static scratch",
+                (long long)ns, (long long)nt, ws_bytes);
+  } else {
+    pl = make_plan(kernel, precision, ns, nt);
+  }
   const size_t need = plan_ws(kernel, precision, pl, nt);
   F3D_REQUIRE(ws != nullptr && ws_bytes >= need, F3D_EWORKSPACE,
               "pairsum: workspace %zu bytes < required %zu", ws_bytes, need);
@@ -733,6 +766,50 @@ extern "C" int f3d_pairsum(int kernel, int precision, uint32_t flags, int64_t ns
   return pairsum_impl(kernel, precision, flags, ns, src_pos, src_nrm, src_den, src_w, s_sp, s_sc,
                       nt, trg_pos, t_sp, t_sc, excl_lo, excl_hi, eta, reg, eps, out, o_sp, o_sc,
                       ws, ws_bytes, nullptr, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free pair sum: the workspace is a static device array, one copy
+// per GPU.  Its address on the current device is taken once with
+// hipGetSymbolAddress (the array's name is only a host-side stand-in) and
+// kept per device.  Nothing in it has to start at zero: every call writes
+// the records, partial sums and bounding box it reads.
+namespace f3d {
+constexpr size_t kScratchBytes = size_t(F3D_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_for_current_device(void** out) {
+  int dev = -1;
+  F3D_HIP_CHECK(hipGetDevice(&dev));
+  F3D_REQUIRE(dev >= 0 && dev < kMaxDevices, F3D_EINVAL, "pairsum_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    F3D_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    F3D_REQUIRE(p != nullptr, F3D_EHIP, "pairsum_noalloc: no scratch address on device %d", dev);
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return F3D_OK;
+}
+}  // namespace f3d
+
+extern "C" int f3d_pairsum_noalloc(int kernel, int precision, uint32_t flags, int64_t ns,
+                                   const double* src_pos, const double* src_nrm,
+                                   const double* src_den, const double* src_w, int64_t s_sp,
+                                   int64_t s_sc, int64_t nt, const double* trg_pos, int64_t t_sp,
+                                   int64_t t_sc, const int32_t* excl_lo, const int32_t* excl_hi,
+                                   double eta, double reg, double eps, double* out, int64_t o_sp,
+                                   int64_t o_sc, void* stream) {
+  void* ws = nullptr;
+  const int rc = scratch_for_current_device(&ws);
+  if (rc != F3D_OK) return rc;
+  return pairsum_impl(kernel, precision, flags, ns, src_pos, src_nrm, src_den, src_w, s_sp, s_sc,
+                      nt, trg_pos, t_sp, t_sc, excl_lo, excl_hi, eta, reg, eps, out, o_sp, o_sc,
+                      ws, kScratchBytes, nullptr, stream, true);
 }
 
 extern "C" size_t f3d_periphery_workspace_size(int precision, int64_t n) {

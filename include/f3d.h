@@ -120,6 +120,29 @@ int f3d_pairsum(int kernel, int precision, uint32_t flags,
                 double* out, int64_t o_sp, int64_t o_sc,
                 void* ws, size_t ws_bytes, void* stream);
 
+/*
+ * Allocation-free f3d_pairsum.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as f3d_pairsum, without ws/ws_bytes: the scratch is a static
+ * device array of F3D_NOALLOC_SCRATCH_BYTES per GPU, held from the library's
+ * first call on that GPU until the process ends.  Where the default launch
+ * shape needs more, the sources are split into fewer chunks (the summation
+ * order, hence the rounding, may then differ from f3d_pairsum's); sizes that
+ * do not fit even one chunk (source records plus 24 bytes per target) are
+ * rejected with F3D_EWORKSPACE.  Nothing is allocated or freed.  Every call
+ * on one GPU shares that GPU's scratch: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller.
+ */
+#define F3D_NOALLOC_SCRATCH_BYTES (64u << 20)
+int f3d_pairsum_noalloc(int kernel, int precision, uint32_t flags,
+                        int64_t ns, const double* src_pos, const double* src_nrm,
+                        const double* src_den, const double* src_w, int64_t s_sp, int64_t s_sc,
+                        int64_t nt, const double* trg_pos, int64_t t_sp, int64_t t_sc,
+                        const int32_t* excl_lo, const int32_t* excl_hi,
+                        double eta, double reg, double eps,
+                        double* out, int64_t o_sp, int64_t o_sc, void* stream);
+
 /* ---- periphery (confining shell) operator ---------------------------- */
 
 size_t f3d_periphery_workspace_size(int precision, int64_t n);

@@ -53,6 +53,9 @@ def test_bad_arguments_rejected_on_host():
     rc = lib.f3d_pairsum(L.OSEEN, L.F64, 64, 10, null, null, null, null, 3, 1, 10, null, 3, 1,
                          null, null, 1.0, 5e-3, 1e-5, null, 3, 1, null, 0, null)
     assert rc == L.EINVAL and b"flags" in lib.f3d_last_error()
+    rc = lib.f3d_pairsum_noalloc(7, L.F64, 0, 10, null, null, null, null, 3, 1, 10, null, 3, 1,
+                                 null, null, 1.0, 5e-3, 1e-5, null, 3, 1, null)
+    assert rc != L.OK
     rc = lib.f3d_lu_factor_batched(L.F64, 4, 300, null, null, null, null, null, null, null,
                                    null, null, null, null, null, null)
     assert rc == L.EINVAL and b"nmax" in lib.f3d_last_error()
