@@ -70,14 +70,47 @@ def _alpha(N, discretisation):
     return -np.cos(np.pi * np.arange(N) / (N - 1))
 
 
+class _NodeIndex:
+    """Nearest-node queries over the nodes placed so far: a k-d tree of the
+    bulk (rebuilt every 16384 nodes), one of the recent nodes (rebuilt every
+    512) and a brute-force check of the newest few hundred."""
+
+    def __init__(self):
+        self.bulk = np.zeros((0, 3))
+        self.recent = np.zeros((0, 3))
+        self.pending = []
+        self.t_bulk = None
+        self.t_recent = None
+
+    def too_close(self, x, sep):
+        for t in (self.t_bulk, self.t_recent):
+            if t is not None and t.query(x, k=1, distance_upper_bound=sep)[0].min() < sep:
+                return True
+        if self.pending:
+            P = np.concatenate(self.pending)
+            if np.linalg.norm(x[:, None, :] - P[None, :, :], axis=2).min() < sep:
+                return True
+        return False
+
+    def add(self, x):
+        self.pending.append(x)
+        if sum(p.shape[0] for p in self.pending) > 512:
+            self.recent = np.concatenate([self.recent] + self.pending)
+            self.pending = []
+            if self.recent.shape[0] > 16384:
+                self.bulk = np.concatenate([self.bulk, self.recent])
+                self.recent = np.zeros((0, 3))
+                self.t_bulk = cKDTree(self.bulk)
+                self.t_recent = None
+            else:
+                self.t_recent = cKDTree(self.recent)
+
+
 def make_fibers(spec: FiberSpec, seed: int, discretisation="reference"):
     """List of (N_k, 3) node arrays placed without node-node overlap."""
     rng = np.random.default_rng(seed)
     placed = []
-    pts = np.zeros((0, 3))
-    tree = None
-    tree_n = 0
-    pending = []
+    index = _NodeIndex()
     attempts = 0
     while len(placed) < spec.num_fibers:
         attempts += 1
@@ -93,21 +126,10 @@ def make_fibers(spec: FiberSpec, seed: int, discretisation="reference"):
         c = spec.ball_radius * rng.uniform() ** (1.0 / 3.0) * h / np.linalg.norm(h)
         s = (1.0 + _alpha(N, discretisation)) * spec.length / 2.0
         x = c[None, :] + (s - spec.length / 2.0)[:, None] * t[None, :]
-        ok = True
-        if tree is not None and tree.query(x, k=1, distance_upper_bound=spec.min_sep)[0].min() < spec.min_sep:
-            ok = False
-        if ok and pending:
-            P = np.concatenate(pending)
-            d = np.linalg.norm(x[:, None, :] - P[None, :, :], axis=2)
-            ok = d.min() >= spec.min_sep
-        if not ok:
+        if index.too_close(x, spec.min_sep):
             continue
         placed.append(x)
-        pending.append(x)
-        if sum(p.shape[0] for p in pending) > 4096:
-            pts = np.concatenate([pts] + pending)
-            pending = []
-            tree = cKDTree(pts)
+        index.add(x)
     return placed
 
 
