@@ -553,43 +553,79 @@ __device__ __forceinline__ void permute_rows(double (&w)[kSlots], const int (&sr
   for (int m = 0; m < kSlots; ++m) w[m] = out[m];
 }
 
+// Column streaming for the one-wave solves: column k of a column-major n x n
+// block is consumed in order k = k_begin, k_begin + step, ... (count columns),
+// each lane holding rows lane + 64 m.  The loads of the next kPf columns are
+// in flight while a column is used, so a wave keeps ~kPf columns of HBM
+// traffic outstanding instead of one latency per column.  want(i, k) masks the
+// rows a column contributes (the triangle), so masked lanes load nothing.
+constexpr int kPf = 8;
+
+template <typename T, typename Want, typename Use>
+__device__ __forceinline__ void stream_cols(const T* __restrict__ M, int n, int k_begin, int step,
+                                            int count, int lane, Want&& want, Use&& use) {
+  T ring[kPf][kSlots];
+  auto load = [&](int idx, T (&v)[kSlots]) {
+    const int k = k_begin + idx * step;
+    const T* __restrict__ col = M + (long)k * n;
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m) {
+      const int i = 64 * m + lane;
+      v[m] = (idx < count && i < n && want(i, k)) ? col[i] : (T)0;
+    }
+  };
+#pragma unroll
+  for (int u = 0; u < kPf; ++u) load(u, ring[u]);
+  for (int i0 = 0; i0 < count; i0 += kPf) {
+#pragma unroll
+    for (int u = 0; u < kPf; ++u) {
+      const int idx = i0 + u;
+      if (idx < count) {
+        T v[kSlots];
+#pragma unroll
+        for (int m = 0; m < kSlots; ++m) v[m] = ring[u][m];
+        load(idx + kPf, ring[u]);
+        use(k_begin + idx * step, v);
+      }
+    }
+  }
+}
+
+// w[slot] of lane (k % 64), slot k / 64 (k wave-uniform)
+__device__ __forceinline__ double pick_row(const double (&w)[kSlots], int k) {
+  const int ms = k >> 6;
+  double v = w[0];
+#pragma unroll
+  for (int m = 1; m < kSlots; ++m) v = (ms == m) ? w[m] : v;
+  return readlane_f64(v, k & 63);
+}
+
+// In-register LU solve of one block held by one wave: w <- U^-1 L^-1 w (the
+// factor's L unit lower, U upper, column-major), FP64 arithmetic.
 template <typename T>
 __device__ __forceinline__ void trsv_lu(const T* __restrict__ L, int n, int lane, double (&w)[kSlots]) {
-  const int mc = (n + 63) / 64;
-  // forward: unit lower triangular
+  // forward: w_i -= L(i, k) w_k for i > k
+  stream_cols(L, n, 0, 1, n - 1, lane, [](int i, int k) { return i > k; },
+              [&](int k, const T (&c)[kSlots]) {
+                const double wk = pick_row(w, k);
 #pragma unroll
-  for (int m = 0; m < kSlots; ++m) {
-    if (m >= mc) break;
-    const int kend = min(64, n - 64 * m);
-#pragma unroll 4
-    for (int kk = 0; kk < kend; ++kk) {
-      const int k = 64 * m + kk;
-      const double wk = readlane_f64(w[m], kk);
-      const T* __restrict__ col = L + (long)k * n;
+                for (int m = 0; m < kSlots; ++m) w[m] = fma(-(double)c[m], wk, w[m]);
+              });
+  // backward: w_k /= U(k, k); w_i -= U(i, k) w_k for i < k
+  stream_cols(L, n, n - 1, -1, n, lane, [](int i, int k) { return i <= k; },
+              [&](int k, const T (&c)[kSlots]) {
+                const int ms = k >> 6;
+                double d = (double)c[0];
 #pragma unroll
-      for (int mm = m; mm < kSlots; ++mm) {
-        const int i = 64 * mm + lane;
-        if (mm < mc && i > k && i < n) w[mm] = fma(-(double)col[i], wk, w[mm]);
-      }
-    }
-  }
-  // backward: upper triangular
+                for (int m = 1; m < kSlots; ++m) d = (ms == m) ? (double)c[m] : d;
+                const double ukk = readlane_f64(d, k & 63);
+                const double wk = pick_row(w, k) / ukk;
 #pragma unroll
-  for (int m = kSlots - 1; m >= 0; --m) {
-    if (m >= mc) continue;
-    const int kend = min(64, n - 64 * m);
-    for (int kk = kend - 1; kk >= 0; --kk) {
-      const int k = 64 * m + kk;
-      const T* __restrict__ col = L + (long)k * n;
-      if (lane == kk) w[m] = w[m] / (double)col[k];
-      const double wk = readlane_f64(w[m], kk);
-#pragma unroll
-      for (int mm = 0; mm <= m; ++mm) {
-        const int i = 64 * mm + lane;
-        if (i < k) w[mm] = fma(-(double)col[i], wk, w[mm]);
-      }
-    }
-  }
+                for (int m = 0; m < kSlots; ++m) {
+                  const int i = 64 * m + lane;
+                  w[m] = (i == k) ? wk : ((i < k) ? fma(-(double)c[m], wk, w[m]) : w[m]);
+                }
+              });
 }
 
 template <typename T>
@@ -645,26 +681,16 @@ lu_solve_kernel(const int* __restrict__ n_arr, const long* __restrict__ a_off,
       r[m] = xv[m];
       rowabs[m] = 0.0;
     }
-    const int mc = (n + 63) / 64;
+    const bool first = anorm < 0.0;
+    stream_cols(Ab, n, 0, 1, n, lane, [](int, int) { return true; },
+                [&](int j, const double (&c)[kSlots]) {
+                  const double yj = pick_row(yv, j);
 #pragma unroll
-    for (int m0 = 0; m0 < kSlots; ++m0) {
-      if (m0 >= mc) break;
-      const int kend = min(64, n - 64 * m0);
-      for (int kk = 0; kk < kend; ++kk) {
-        const int j = 64 * m0 + kk;
-        const double yj = readlane_f64(yv[m0], kk);
-        const double* __restrict__ col = Ab + (long)j * n;
-#pragma unroll
-        for (int mm = 0; mm < kSlots; ++mm) {
-          const int i = 64 * mm + lane;
-          if (mm < mc && i < n) {
-            const double aij = col[i];
-            r[mm] = fma(-aij, yj, r[mm]);
-            if (anorm < 0.0) rowabs[mm] += fabs(aij);
-          }
-        }
-      }
-    }
+                  for (int mm = 0; mm < kSlots; ++mm) {
+                    r[mm] = fma(-c[mm], yj, r[mm]);
+                    if (first) rowabs[mm] += fabs(c[mm]);
+                  }
+                });
     double rn = 0.0, yn = 0.0, an = 0.0;
 #pragma unroll
     for (int m = 0; m < kSlots; ++m) {
@@ -807,6 +833,11 @@ static int num_cus() {
   return cached;
 }
 
+int launch_lu_wave(int precision, int64_t nb, const int32_t* n, const int64_t* a_off,
+                   const int64_t* x_off, const int32_t* order, const double* A, double* lu64,
+                   float* lu32, double* rs, double* cs, int32_t* piv, int32_t* perm,
+                   int32_t* info, int num_cu, hipStream_t st);
+
 }  // namespace f3d
 
 using namespace f3d;
@@ -825,6 +856,19 @@ extern "C" int f3d_lu_factor_batched(int precision, int64_t nb, int32_t nmax, co
   F3D_REQUIRE(n && a_off && x_off && A && piv && perm && info, F3D_EINVAL,
               "lu_factor: NULL argument");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  // default: one wavefront per block, blocked, MFMA trailing update
+  // (lu_wave.hip); F3D_LU_KERNEL=workgroup selects the one-workgroup-per-block
+  // kernels below (FP32: F3D_LU32_KERNEL=vector for the unblocked one)
+  const char* sel_k = getenv("F3D_LU_KERNEL");
+  if (!(sel_k && strcmp(sel_k, "workgroup") == 0)) {
+    if (precision == F3D_F64) {
+      F3D_REQUIRE(lu64 != nullptr, F3D_EINVAL, "lu_factor: lu64 is NULL");
+    } else {
+      F3D_REQUIRE(lu32 && rs && cs, F3D_EINVAL, "lu_factor: lu32/rs/cs is NULL");
+    }
+    return launch_lu_wave(precision, nb, n, a_off, x_off, order, A, lu64, lu32, rs, cs, piv,
+                          perm, info, num_cus(), st);
+  }
   const unsigned grid = (unsigned)(nb < num_cus() ? nb : num_cus());
   if (precision == F3D_F64) {
     F3D_REQUIRE(lu64 != nullptr, F3D_EINVAL, "lu_factor: lu64 is NULL");
