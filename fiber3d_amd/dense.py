@@ -7,8 +7,11 @@ tstep_utils.build_block_diagonal_lu_preconditioner_fiber
 (python/tstep/tstep_utils.py:1405-1419) and the lu_solve loop of tstep.P_inv
 (python/tstep/tstep.py:985-988).
 """
+import warnings
+
 import numpy as np
 import torch
+from scipy.linalg import LinAlgWarning
 
 from . import _lib as L
 from .kernels import _stream, device
@@ -77,6 +80,13 @@ class BatchedLU:
         self.info = torch.empty(blocks.nb, dtype=torch.int32, device=dev)
         self.iters = torch.zeros(blocks.nb, dtype=torch.int32, device=dev)
         self.factor()
+
+    def singular_blocks(self):
+        """(block, 1-based index of the first exactly-zero pivot) for every
+        singular block, from the info array of the last factorisation (one
+        device read)."""
+        info = self.info.cpu().numpy()
+        return [(int(b), int(k)) for b, k in enumerate(info) if k > 0]
 
     def factor(self):
         bs = self.bs
@@ -174,6 +184,10 @@ def build_block_diagonal_lu_preconditioner_fiber(fibers, A_fibers_blocks):
         return [], []
     bs = BlockSet([A.shape[0] for A in mats])
     lu = BatchedLU(bs, bs.pack(mats), precision="f64")
+    for b, k in lu.singular_blocks():
+        # scipy.linalg.lu_factor's warning for an exactly-zero pivot
+        warnings.warn(f"Diagonal number {k} is exactly zero. Singular matrix (fiber {b}).",
+                      LinAlgWarning, stacklevel=2)
     LU_all = bs.unpack(lu.lu64)
     piv = lu.piv.cpu().numpy()
     P_all = [piv[o:o + n].astype(np.int32) for o, n in zip(bs.x_off_h, bs.sizes)]

@@ -24,8 +24,11 @@ force/torque-free ends inside a spherical periphery:
 Unknown vector layout is the reference's: [shell density (3 Nper, AoS) |
 per body density (3 Nb, AoS), U, Omega | per fiber X(N), Y(N), Z(N), T(N)].
 """
+import warnings
+
 import numpy as np
 import torch
+from scipy.linalg import LinAlgWarning
 
 from . import _lib as L
 from . import fiber_matrices as FMX
@@ -747,11 +750,20 @@ class FiberSystem:
             self._apply_clamped()
         if self.hng:
             self._apply_hinged()
+        factored = True
         if getattr(self, "lu", None) is None:
             self.lu = BatchedLU(self.bs, self.A_buf, precision=self.lu_precision,
                                 refine=self.lu_refine, tol=self.lu_tol)
         elif refactor:
             self.lu.factor()
+        else:
+            factored = False
+        if factored and int(self.lu.info.max()) > 0:
+            # scipy.linalg.lu_factor warns on an exactly-zero pivot; one device
+            # read per factorisation
+            bad = self.lu.singular_blocks()
+            warnings.warn(f"{len(bad)} fiber block(s) exactly singular (fiber, first zero "
+                          f"pivot): {bad[:8]}", LinAlgWarning, stacklevel=2)
 
     def _assemble_hip(self, A_buf, F_buf, Q_buf):
         """A_sq, force rows, tension projections, tangents and weights of every

@@ -208,3 +208,21 @@ def test_vertex_and_links_files(tmp_path):
     np.testing.assert_array_equal(ka, [[5.0], [0.5]])
     np.testing.assert_array_equal(loc, [[0, 0, 0.5], [0.3, 0, 0.4]])
     np.testing.assert_allclose(ax, [[0, 0, 1], [0.6, 0.8, 0]], rtol=0, atol=1e-15)
+
+
+def test_stepper_from_input_rejects_unset_stencil_and_missing_length(tmp_path):
+    """stepper_from_input needs num_points_finite_diff (the reference would
+    build one-point stencils from an unset 0) and a positive length for every
+    fiber; both are checked before any device work."""
+    from fiber3d_amd.fileio import ReadInput
+    from fiber3d_amd.tstep import stepper_from_input
+    f = tmp_path / "data.main"
+    f.write_text("dt 1e-3\neta 1.0\nE 0.5\nstructure_fibers %s\n"
+                 % os.path.join(GOLDEN, "two.fibers"))
+    with pytest.raises(ValueError, match="num_points_finite_diff"):
+        stepper_from_input(ReadInput(str(f)), None)
+    fib = tmp_path / "nolen.fibers"
+    fib.write_text("1\n5 0.5\n" + "".join(f"{0.1 * k} 0 0\n" for k in range(5)))
+    f.write_text("dt 1e-3\nnum_points_finite_diff 1\nstructure_fibers %s\n" % fib)
+    with pytest.raises(ValueError, match="positive length"):
+        stepper_from_input(ReadInput(str(f)), None)

@@ -96,6 +96,7 @@ def test_stepper_from_input_file(tmp_path):
     from fiber3d_amd.tstep import stepper_from_input
     f = tmp_path / "data.main"
     f.write_text("dt 1e-3\neta 1.0\nsolver_tolerance 1e-10\nE 0.5\n"
+                 "num_points_finite_diff 4\n"
                  "structure_fibers %s\n" % os.path.join(GOLDEN, "two.fibers"))
     r = ReadInput(str(f))
     xsh, nsh, wsh = configs.make_periphery(800, 8.0)
