@@ -48,8 +48,7 @@ import torch
 
 from .fileio import (fibers_from_file, write_body_velocities_frame, write_clones_frame,
                      write_fibers_frame, write_resume_clones, write_resume_fibers)
-from .gmres import gmres
-from .graphs import ArnoldiGraphs
+from .gmres import GmresContext, gmres
 from .system import FiberSystem
 
 
@@ -150,7 +149,8 @@ class TimeStepper:
         # a surface, large cancelling forces of a fiber held at both ends),
         # so it is measured on the corrections of guess-started solves,
         # free of cost whenever a solve closes with the FP64 residual.  When
-        # no solve has for 2 lu_every steps, the next solve that iterates is
+        # no solve has for 2 lu_every steps, and after every re-factorisation,
+        # change of dt or rejected step, the next solve that iterates is
         # closed in FP64 to measure it (one FP64 product).  The bound uses
         # the largest ratio of the last 4 lu_every steps times a safety
         # factor, never below the floor; until the first measurement every
@@ -159,6 +159,7 @@ class TimeStepper:
         self.inner_rel_err_floor = 1e-6
         self.inner_rel_err_safety = 10.0
         self._inner_meas = []    # (step index, measured ratio)
+        self._changed = False    # the last assembly re-factored or changed dt
         self.adaptive = adaptive
         self.tol_tstep, self.dt_min, self.dt_max = tol_tstep, dt_min, dt_max
         self.periphery_radius = periphery_radius
@@ -185,8 +186,9 @@ class TimeStepper:
                                cortex=cortex)
         self.predictor = predictor
         S = self.sys
-        # recorded A P^-1 per operator precision, re-recorded after each assembly
-        self.graphs = ArnoldiGraphs(S) if graph else None
+        self.graphs = bool(graph)
+        self.gctx = None
+        self._last_its = 0
         dev = S.dev
         # solution index of every node coordinate: X(N), Y(N), Z(N), T(N) per fiber
         idx = np.empty((S.npts, 3), dtype=np.int64)
@@ -242,14 +244,18 @@ class TimeStepper:
         x0 = None
         if self.predictor is not None:
             x0 = self.predictor.predict(self)
-        fused = {}
-        if self.graphs is not None:
-            G = self.graphs
-            fused["fused_matvec"] = lambda v, out: G.get("f64")(v, out=out)
-            if self.mode == "fast":
-                fused["fused_inner"] = lambda v, out: G.get("f32c")(v, out=out)
+        # device GMRES: buffers (basis, state) reused by every step; with
+        # graph=True each Arnoldi step (basis vector -> A P^-1 -> CGS2 ->
+        # Givens) replays one recorded HIP graph per operator precision
+        if self.gctx is None:
+            self.gctx = GmresContext(S.size, min(self.restart, S.size), S.dev)
+        fused = {"ctx": self.gctx, "expect": max(0, self._last_its - 1)}
+        if self.graphs:
+            fused["graph_keys"] = {"matvec": "f64", "inner": "f32c"}
         certify_update = self.mode == "fast" and self.certify == "update"
-        measure_due = (not self._inner_meas
+        # after a re-factorisation, a change of dt or a rejected step the
+        # measured error may no longer describe the operator: close in FP64
+        measure_due = (not self._inner_meas or self._changed
                        or self.nsteps - self._inner_meas[-1][0] >= 2 * self.lu_every)
         res = gmres(S.matvec, S.rhs, S.precond, tol=self.tol, restart=self.restart,
                     maxiter=self.maxiter, x0=x0, inner=self.inner_operator(),
@@ -258,6 +264,7 @@ class TimeStepper:
                     measure_inner=certify_update, **fused)
         if certify_update and res.guess_used and res.inner_err is not None:
             self._note_inner(res.inner_err)
+        self._last_its = res.iterations
         sol = res.x
         body_old = S.bodies.state() if S.bodies is not None else None
         if self.record:
@@ -273,7 +280,9 @@ class TimeStepper:
         # re-assembly refills the system's buffers in place: the recorded
         # Arnoldi graphs stay valid
         self.nsteps += 1
-        S.prepare(refactor=self.nsteps % self.lu_every == 0)
+        refactor = self.nsteps % self.lu_every == 0
+        S.prepare(refactor=refactor)
+        self._changed = refactor
         err = self.inextensibility_error()
         accept, dt_new = True, dt
         if self.adaptive:
@@ -297,6 +306,7 @@ class TimeStepper:
         if not accept or dt_new != dt:
             S.dt = dt_new
             S.prepare(refactor=True)
+            self._changed = True
         torch.cuda.synchronize()
         st = StepStats(res.iterations, res.restarts, res.residual, time.perf_counter() - t0,
                        res.guess_used, res.residual_kind, res.residual_bound, accept, dt, err)
@@ -385,10 +395,19 @@ def stepper_from_input(read, shell, **kw):
         Ls += L
     if not fibers:
         raise ValueError("the input file names no structure_fibers file")
+    bad = [k for k, L in enumerate(Ls) if not L > 0.0]
+    if bad:
+        raise ValueError(f"fibers {bad[:8]} have no positive length: give it in the fiber "
+                         "file header or as length_fibers in the input file")
     if len(set(Es)) != 1:
         raise ValueError("fibers with different E are not supported: %s" % sorted(set(Es)))
+    if not read.num_points_finite_diff and "nfd" not in kw:
+        # the reference passes the option unchanged to fiber_matrices
+        # (tstep.py:262), where 0 builds one-point stencils (D_1 = 0)
+        raise ValueError("num_points_finite_diff is unset (0) in the input file: set it, or "
+                         "pass nfd=")
     args = dict(dt=read.dt, eta=read.eta, E=Es[0], epsilon=read.epsilon,
-                nfd=read.num_points_finite_diff or 4, tol=read.solver_tolerance,
+                nfd=read.num_points_finite_diff, tol=read.solver_tolerance,
                 lengths=np.asarray(Ls, dtype=np.float64))
     args.update(kw)
     return TimeStepper(fibers, shell, **args)

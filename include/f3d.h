@@ -286,6 +286,36 @@ int f3d_block_gemv_batched(int64_t nb, int32_t mmax, const int32_t* m, const int
 int f3d_dense_gemv_batched(int64_t count, int32_t m, const double* M, const double* x,
                            double* y, void* stream);
 
+/* ---- restarted GMRES on the device ----------------------------------- */
+/*
+ * Replace the host side of the reference's GMRES (SciPy's restarted gmres
+ * under bpm_utilities/gmres.py:94-108, called at python/tstep/tstep.py:1209-
+ * 1216): the Arnoldi orthogonalisation (classical Gram-Schmidt with one
+ * re-orthogonalisation), the Hessenberg matrix, its Givens rotations and the
+ * least-squares residual estimate, all FP64 in device memory.  n is the
+ * system size, m (1..159) the restart length; V is the (m+1) x n basis
+ * (row k = basis vector k, contiguous); ws is zero-filled once, of
+ * f3d_gmres_workspace_size(n, m) bytes, and holds the state of one solve at
+ * a time, including the current step j, so a sequence of calls can be
+ * recorded in a HIP graph and replayed for every step.
+ *   begin:   beta = ||r||, V_0 = r / beta, g = beta e_1, j = 0.
+ *   current: xin = V_j (the operator's input).
+ *   arnoldi: given w = A P^-1 V_j, orthogonalise w against V_0..V_j (CGS2),
+ *            store column j of the Hessenberg matrix (rotated and not),
+ *            rotate, update g, record est_j = |g_{j+1}| (ws, see gmres.hip),
+ *            V_{j+1} = w / h_{j+1,j}, j += 1.  w is overwritten.
+ *   solve:   y = H_k^-1 g_k over the first k steps, upd = V_k y; with r
+ *            non-NULL also r_upd = r - V_{k+1} (H_raw y), the residual the
+ *            Arnoldi relation gives for the correction.
+ */
+size_t f3d_gmres_workspace_size(int64_t n, int32_t m);
+int f3d_gmres_begin(int64_t n, int32_t m, const double* r, double* V, void* ws, void* stream);
+int f3d_gmres_current(int64_t n, int32_t m, const double* V, double* xin, void* ws,
+                      void* stream);
+int f3d_gmres_arnoldi(int64_t n, int32_t m, double* V, double* w, void* ws, void* stream);
+int f3d_gmres_solve(int64_t n, int32_t m, int32_t k, const double* V, double* upd,
+                    const double* r, double* r_upd, void* ws, void* stream);
+
 /* ---- BF16 MFMA GEMM for the learned initial-guess network ------------ */
 /*
  * No counterpart in the reference (the learned GMRES initial guess is new).

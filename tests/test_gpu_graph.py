@@ -80,10 +80,10 @@ def test_graph_time_steps_match_eager_steps():
         assert all(st.residual <= 1e-10 for st in stats)
         runs[graph] = ts.sys.X.cpu().numpy()
         if graph:
-            # recorded once, reused by every later step (assembly refills the
-            # recorded buffers in place)
-            assert 1 <= ts.graphs.captures <= 2
-            assert sum(op.replays for op in ts.graphs._ops.values()) >= 3 * 2
+            # each Arnoldi step's graph (one per operator precision) is
+            # recorded once and replayed by every later step (assembly refills
+            # the recorded buffers in place)
+            assert 1 <= ts.gctx.captures <= 2
     assert rel2(runs[True], runs[False]) < 1e-12
 
 
@@ -104,24 +104,24 @@ def test_graph_scratch_outlives_recording():
 
 
 def test_graph_of_attached_system_survives_steps():
-    """A recorded A P^-1 of a system with fibers clamped to bodies stays equal
-    to the eager operator after steps re-assemble it (the clamped rows, the
-    tangents and the links are refilled in place)."""
+    """Recorded Arnoldi steps of a system with fibers clamped to bodies stay
+    equal to eager ones after steps re-assemble it (the clamped rows, the
+    tangents and the links are refilled in place): the trajectories agree."""
     print("case graph attached", flush=True)
     from test_gpu_attached import _setup as attached_setup
     fibers, bodies, attached, (xs, ns, ws) = attached_setup()
     force = [np.array([0.5, 0.0, 0.0, 0.0, 0.0, 0.3]), np.zeros(6)]
-    ts = TimeStepper(fibers, Shell(xs, ns, ws), mode="fast", tol=1e-10, bodies=bodies,
-                     body_force=force, attached=attached, graph=True)
-    S = ts.sys
-    rng = np.random.default_rng(8)
-    for _ in range(3):
-        st = ts.step()
-        assert st.residual <= 1e-10
-        for precision in ("f32c", "f64"):
-            v = torch.from_numpy(rng.normal(size=S.size)).cuda()
-            ref = S.matvec(S.precond(v), precision=precision)
-            got = ts.graphs.get(precision)(v)
-            torch.cuda.synchronize()
-            assert rel2(got.cpu().numpy(), ref.cpu().numpy()) < 1e-13, precision
-    assert 1 <= ts.graphs.captures <= 2
+    runs = {}
+    for graph in (False, True):
+        ts = TimeStepper(fibers, Shell(xs, ns, ws), mode="fast", tol=1e-10, bodies=bodies,
+                         body_force=force, attached=attached, graph=graph)
+        for _ in range(4):
+            assert ts.step().residual <= 1e-10
+        torch.cuda.synchronize()
+        runs[graph] = ts.sys.X.cpu().numpy()
+        if graph:
+            assert 1 <= ts.gctx.captures <= 2
+    # the body rows sum over surface nodes with atomic adds (index_add_), so
+    # two runs of this system differ at rounding level whether recorded or
+    # not; solves to 1e-10 then agree to about the conditioning times that
+    assert rel2(runs[True], runs[False]) < 1e-7
