@@ -7,11 +7,14 @@
 //   rounded to BF16 -- round to nearest even -- when staged into LDS);
 //   C(m, n) = C[m*scm + n*scn] (FP32).
 //
-// Tiling for CDNA4: a 256-thread workgroup owns a 64 x 64 tile of C; each
-// wave owns a 32 x 32 quarter and issues v_mfma_f32_32x32x16_bf16 on 16-deep
+// General tiling for CDNA4 (any shape, K splits): a 256-thread workgroup owns
+// a 64 x 64 tile of C; each wave owns a 32 x 32 quarter and issues
+// v_mfma_f32_32x32x16_bf16 on 16-deep
 // K slices staged through LDS (A row-major, B transposed so both fragments
 // are 16-byte contiguous LDS reads).  grid.z splits K for the weight-gradient
 // products (K = number of nodes), whose tiles are then added atomically.
+// The per-node layers (N = 128 or 256, K <= 256) take the streaming kernel
+// below.
 #include "common.h"
 
 namespace f3d {
@@ -100,6 +103,132 @@ gemm_bf16_kernel(int M, int N, int K, int kchunk, const float* __restrict__ A, l
   }
 }
 
+// ---------------------------------------------------------------------------
+// Streaming path for the per-node layers (M = nodes, large; N a multiple of
+// 64 up to 256; K <= 256): a workgroup owns 128 rows and ALL N columns, so
+// every activation row is read from HBM once and written once.  Four waves,
+// 2 x 2, each owning 64 rows x N/2 columns as (2 x N/64) 32x32 MFMA tiles
+// (up to 128 accumulator registers).  K is staged through LDS in 32-deep
+// chunks, double buffered: the next chunk's global loads are issued before
+// the current chunk's MFMAs, one barrier per chunk; each 16-deep step
+// reuses every A fragment across the wave's column tiles and every B
+// fragment across its two row tiles.
+constexpr int kSM = 128;  // rows per workgroup
+constexpr int kSK = 32;   // K chunk
+constexpr int kSP = kSK + 8;  // LDS row pitch (bf16 elements): 80 B, 16-byte aligned
+
+template <int EPI, int NT>  // NT = N / 64 column tiles per wave (1..2 pairs -> N = 64 NT)
+__global__ void __launch_bounds__(256)
+gemm_bf16_stream_kernel(int M, int N, int K, const float* __restrict__ A, long sam,
+                        const float* __restrict__ B, long sbk, long sbn,
+                        float* __restrict__ C, long scm, const float* __restrict__ bias,
+                        float* __restrict__ Z, const float* __restrict__ Zin, float alpha) {
+  constexpr int NW = 64 * NT;  // columns of the workgroup = N
+  __shared__ __attribute__((aligned(16))) unsigned short As[2][kSM][kSP];
+  __shared__ __attribute__((aligned(16))) unsigned short Bt[2][NW][kSP];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.x * kSM;
+  f32x16_t acc[2][NT];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NT; ++b)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.0f;
+
+  // staging: A chunk 128 x 32 (16 floats per thread: row tid/2, k half
+  // (tid&1)*16), B chunk 32 x NW (NW*32/256 = 8 NT floats per thread)
+  float ra[16], rb[8 * NT];
+  auto load = [&](int k0) __attribute__((always_inline)) {
+    const int r = tid >> 1, kh = (tid & 1) * 16;
+    const int m = m0 + r;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int k = k0 + kh + e;
+      ra[e] = (m < M && k < K) ? A[(long)m * sam + k] : 0.0f;
+    }
+#pragma unroll
+    for (int e = 0; e < 8 * NT; ++e) {
+      const int idx = tid + 256 * e;  // over (k, n): n fastest
+      const int kk = idx / NW, n = idx - kk * NW;
+      const int k = k0 + kk;
+      rb[e] = (k < K && n < N) ? B[(long)k * sbk + (long)n * sbn] : 0.0f;
+    }
+  };
+  auto store = [&](int buf) __attribute__((always_inline)) {
+    const int r = tid >> 1, kh = (tid & 1) * 16;
+#pragma unroll
+    for (int e = 0; e < 16; e += 2) {
+      const unsigned v = (unsigned)f2bf(ra[e]) | ((unsigned)f2bf(ra[e + 1]) << 16);
+      *reinterpret_cast<unsigned*>(&As[buf][r][kh + e]) = v;
+    }
+#pragma unroll
+    for (int e = 0; e < 8 * NT; ++e) {
+      const int idx = tid + 256 * e;
+      const int kk = idx / NW, n = idx - kk * NW;
+      Bt[buf][n][kk] = f2bf(rb[e]);
+    }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  const int nchunks = (K + kSK - 1) / kSK;
+  for (int c = 0; c < nchunks; ++c) {
+    const int buf = c & 1;
+    if (c + 1 < nchunks) load((c + 1) * kSK);  // in flight during the MFMAs
+#pragma unroll
+    for (int ks = 0; ks < kSK; ks += 16) {
+      const int kofs = ks + 8 * (lane >> 5);
+      bf16x8_t fa[2], fb[NT];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+        fa[a] = __builtin_bit_cast(
+            bf16x8_t, *reinterpret_cast<const uint4*>(&As[buf][64 * wm + 32 * a + (lane & 31)][kofs]));
+#pragma unroll
+      for (int b = 0; b < NT; ++b)
+        fb[b] = __builtin_bit_cast(
+            bf16x8_t, *reinterpret_cast<const uint4*>(&Bt[buf][NW / 2 * wn + 32 * b + (lane & 31)][kofs]));
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    if (c + 1 < nchunks) store(buf ^ 1);
+    __syncthreads();
+  }
+  // epilogue: register i of lane l holds row 8(i/4) + 4(l/32) + i%4, column l%32
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NT; ++b) {
+      const int n = NW / 2 * wn + 32 * b + (lane & 31);
+      const float bn = (bias && n < N) ? bias[n] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int m = m0 + 64 * wm + 32 * a + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        if (m >= M || n >= N) continue;
+        float v = alpha * acc[a][b][i];
+        const long ci = (long)m * scm + n;
+        if (EPI == EPI_BIAS_RELU || EPI == EPI_BIAS) {
+          v += bn;
+          if (EPI == EPI_BIAS_RELU) {
+            if (Z) Z[ci] = v;
+            v = v > 0.0f ? v : 0.0f;
+          }
+          C[ci] = v;
+        } else if (EPI == EPI_RELU_GRAD) {
+          C[ci] = Zin[ci] > 0.0f ? v : 0.0f;
+        } else {
+          C[ci] = v;
+        }
+      }
+    }
+}
+
 }  // namespace f3d
 
 using namespace f3d;
@@ -121,6 +250,24 @@ extern "C" int f3d_gemm_bf16(int epi, int64_t M, int64_t N, int64_t K, int32_t k
   if (M == 0 || N == 0) return F3D_OK;
   F3D_REQUIRE(A && B && C, F3D_EINVAL, "gemm_bf16: NULL operand");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  // the streaming path: row-major A (unit K stride) and C (unit N stride),
+  // N in {128, 256}, K <= 256, no K split
+  const bool stream_ok = epi != EPI_ATOMIC && ksplit == 1 && sak == 1 && scn == 1 &&
+                         (N == 128 || N == 256) && K <= 256 && M >= kSM;
+  if (stream_ok) {
+    const dim3 g((unsigned)ceil_div(M, kSM));
+#define F3D_S(E, T)                                                                            F3D_LAUNCH((gemm_bf16_stream_kernel<E, T>), g, dim3(256), 0, st, (int)M, (int)N, (int)K, A,              (long)sam, B, (long)sbk, (long)sbn, C, (long)scm, bias, Z, Zin, alpha)
+#define F3D_SE(E) if (N == 256) F3D_S(E, 4); else F3D_S(E, 2)
+    switch (epi) {
+      case EPI_STORE: F3D_SE(EPI_STORE); break;
+      case EPI_BIAS_RELU: F3D_SE(EPI_BIAS_RELU); break;
+      case EPI_RELU_GRAD: F3D_SE(EPI_RELU_GRAD); break;
+      default: F3D_SE(EPI_BIAS); break;
+    }
+#undef F3D_SE
+#undef F3D_S
+    return F3D_OK;
+  }
   int64_t kchunk = ceil_div(K > 0 ? K : 1, ksplit);
   kchunk = (int64_t)align_up((size_t)kchunk, kKT);
   const unsigned gz = (unsigned)ceil_div(K > 0 ? K : 1, kchunk);
