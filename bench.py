@@ -14,12 +14,18 @@ numba loop, OpenMP over targets, on a bounded sample of targets).
 "fiber_system" (default: the large workload, 4096 fibers x 64 nodes, a
 32768-node periphery and 8 rigid spheres) reports time-steps/s to the 1e-8
 FP64 GMRES residual for: plain FP64 (the reference method on the GPU), fast
-(FP32-compensated inner matvec, FP32 MFMA fiber LU), extrapolated_fast and
-learned_fast (initial guess from the solver's own past steps).  Each mode
-runs its own trajectory from the same seeded state; untimed warm-up steps,
-including the predictor's training, precede the timed ones.
+(FP32-compensated inner matvec, FP32 MFMA fiber LU), extrapolated_fast,
+corrected_fast (extrapolation of the residual-corrected history) and
+learned_fast (the trained per-node net), all with initial guesses from the
+solver's own past steps.  "fiber_system_series" does the same for the time
+series (1000 fibers x 32 nodes, 8000-node periphery; the learned predictor
+trains on its first 100 steps, then 300 steps are timed).  Each mode runs
+its own trajectory from the same seeded state; untimed warm-up steps,
+including the predictor's training, precede the timed ones.  Both legs
+carry the oracle's CPU time per plain step, extrapolated from samples.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--system 0|3|4|-1]
+       [--no-series] [--no-lu] [--no-small] [--no-cpu] [--no-graph]
 """
 import argparse
 import json
@@ -48,29 +54,133 @@ PEAK_BF16_TFLOPS = 2500.0
 OSEEN_F32C_HBM_BYTES = 86076086
 
 
-def cpu_baseline(x, f, n_targets=4096, cores=None):
-    """Oracle C restatement of kernels.py:794-846 on the host (port)."""
+def host_cores():
+    """Host threads this process may use: the CPU affinity set, capped by the
+    cgroup CPU quota where one is set; with the CPU model and OpenMP
+    settings, for the baseline's report."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    cores = min(n, quota) if quota else n
+    return {"threads": int(cores), "affinity_cpus": int(n), "cgroup_quota_cpus": quota,
+            "cpu_model": model, "OMP_PROC_BIND": "close"}
+
+
+def _omp(cores):
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    os.environ["OMP_PROC_BIND"] = "close"
+
+
+def cpu_baseline(x, f, n_targets=4096):
+    """Oracle C restatement of kernels.py:794-846 on the host (port), on all
+    the host threads this process may use."""
     from oracle import build as obuild
     from oracle import cbaseline
     if not os.path.exists(obuild.OUT):
         obuild.build(verbose=False)
-    cores = cores or min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(cores)
+    host = host_cores()
+    _omp(host["threads"])
     rows = np.arange(n_targets)
     cbaseline.oseen(x, x[:64], f)  # warm-up (thread pool)
     times = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         cbaseline.oseen(x, x[rows], f)
         times.append(time.perf_counter() - t0)
     dt = float(np.median(times))
-    return {"value": round(N_PTS * n_targets / dt / 1e9, 4), "unit": "Gpairs/s", "cores": cores,
-            "kind": "port",
-            "sample": f"Oseen FP64, all {N_PTS} sources x {n_targets} targets, median of 3"}
+    return {"value": round(N_PTS * n_targets / dt / 1e9, 4), "unit": "Gpairs/s",
+            "cores": host["threads"], "kind": "port", "host": host,
+            "sample": f"Oseen FP64, all {N_PTS} sources x {n_targets} targets, median of 5 "
+                      "after one warm-up"}
+
+
+def oracle_step_estimate(cfg, fibers, xs, ns, bodies, iters, sample_targets=2048,
+                         sample_fibers=64):
+    """CPU time of one plain FP64 step of the reference's method at this
+    workload, extrapolated from bounded samples of the oracle on all host
+    threads: the O(N^2) pair sums of one matvec (C restatement, OpenMP over
+    targets, timed on sample_targets targets and scaled to all targets),
+    per-fiber assembly and LAPACK LU (timed on sample_fibers fibers, scaled
+    to all), and the per-fiber LU solves of the preconditioner; one step =
+    assembly + LU + (iters + 2) x (matvec + preconditioner), iters being the
+    plain GPU solve's iteration count (the same Krylov method from zero)."""
+    import scipy.linalg as sla
+    from oracle import build as obuild
+    from oracle import cbaseline as C
+    from oracle import fiber as OF
+    if not os.path.exists(obuild.OUT):
+        obuild.build(verbose=False)
+    host = host_cores()
+    _omp(host["threads"])
+    spec = cfg.fibers
+    X = np.concatenate(fibers)
+    npts, nper = X.shape[0], xs.shape[0]
+    bn = np.concatenate([b[1] for b in bodies]) if bodies else np.zeros((0, 3))
+    bnn = np.concatenate([b[2] for b in bodies]) if bodies else np.zeros((0, 3))
+    rng = np.random.default_rng(0)
+    f = rng.normal(size=X.shape)
+    q = rng.normal(size=xs.shape)
+    T = X[rng.choice(npts, min(sample_targets, npts), replace=False)]
+    C.oseen(X, T[:16], f)  # warm-up
+    t0 = time.perf_counter()
+    C.oseen(X, T, f)
+    t_os = (time.perf_counter() - t0) * (npts + nper + bn.shape[0]) / T.shape[0]
+    t0 = time.perf_counter()
+    C.stresslet(xs, T, ns, q)
+    t_st = (time.perf_counter() - t0) * (npts + nper + bn.shape[0]) / T.shape[0]
+    t_bd = 0.0
+    if bn.shape[0]:
+        t0 = time.perf_counter()
+        C.stresslet(bn, T, bnn, rng.normal(size=bn.shape))
+        t_bd = (time.perf_counter() - t0) * (npts + nper + bn.shape[0]) / T.shape[0]
+    matvec = t_os + t_st + t_bd
+    ks = rng.choice(len(fibers), min(sample_fibers, len(fibers)), replace=False)
+    c0, c1 = OF.constants(spec.epsilon, cfg.eta)
+    t_as = t_lu = t_sv = 0.0
+    for k in ks:
+        x = fibers[k]
+        fm = OF.FiberMatrices(x.shape[0], cfg.nfd)
+        t0 = time.perf_counter()
+        A = OF.form_linear_operator(fm, x, spec.length, spec.E, cfg.dt, c0, c1, cfg.penalty)
+        R = OF.compute_RHS(fm, x, spec.length, cfg.dt, c0, c1, cfg.penalty)
+        A_sq, b = OF.apply_BC_rectangular(fm, x, spec.length, spec.E, A, R)
+        t1 = time.perf_counter()
+        lu = sla.lu_factor(A_sq, check_finite=False)
+        t2 = time.perf_counter()
+        sla.lu_solve(lu, b, check_finite=False)
+        t3 = time.perf_counter()
+        t_as += t1 - t0
+        t_lu += t2 - t1
+        t_sv += t3 - t2
+    scale = len(fibers) / len(ks)
+    t_as, t_lu, t_sv = t_as * scale, t_lu * scale, t_sv * scale
+    step = t_as + t_lu + (iters + 2) * (matvec + t_sv)
+    return {"value": round(1.0 / step, 6), "unit": "time-steps/s", "cores": host["threads"],
+            "kind": "port", "host": host, "extrapolated": True,
+            "seconds_per_step": round(step, 3),
+            "parts_s": {"pair_sums_per_matvec": round(matvec, 4),
+                        "assembly": round(t_as, 4), "fiber_lu": round(t_lu, 4),
+                        "fiber_lu_solves_per_apply": round(t_sv, 4), "matvecs": iters + 2},
+            "sample": f"oracle on {T.shape[0]} of {npts + nper + bn.shape[0]} targets (pair "
+                      f"sums) and {len(ks)} of {len(fibers)} fibers (assembly, LAPACK LU and "
+                      f"solve), scaled linearly; {iters} GMRES iterations (plain GPU solve)"}
 
 
 def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_bodies=True,
-               graph=False):
+               graph=True, cpu=True):
     """time-steps/s of the implicit fiber step, plain FP64 vs fast (+ learned guess).
 
     Each mode runs its own trajectory from the same seeded initial state:
@@ -108,16 +218,20 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
     modes = [("plain", "plain", None), ("fast", "fast", None)]
     if train_steps:
         modes.append(("extrapolated_fast", "fast", LearnedGuess(train_steps=10**9, use_net=False)))
-        modes.append(("learned_fast", "fast", LearnedGuess(train_steps=train_steps, epochs=40)))
+        modes.append(("corrected_fast", "fast",
+                      LearnedGuess(train_steps=10**9, use_net=False, correct=True)))
+        modes.append(("learned_fast", "fast", LearnedGuess(train_steps=train_steps, epochs=20,
+                                                          max_samples=32768)))
     for name, mode, pred in modes:
         ts = TimeStepper(fibers, shell, mode=mode, eta=cfg.eta, dt=cfg.dt, E=spec.E,
                          epsilon=spec.epsilon, tol=cfg.gmres_tol, restart=cfg.restart,
                          maxiter=cfg.maxiter, predictor=pred, lengths=np.full(len(fibers), spec.length),
                          bodies=bodies, graph=graph, record=mode == "fast")
         # a predictor needs `keep` solutions before its full guess; the learned
-        # mode then collects train_steps samples and trains: all of that
-        # happens before timing starts
-        warm = 1 if pred is None else pred.keep + 2 + (train_steps if pred.use_net else 0)
+        # mode then collects train_steps samples, trains and checks the net on
+        # 4 further steps, all before timing starts.  Every guessing mode times the same steps: those after the
+        # learned mode's history and training window (held-out steps).
+        warm = 1 if pred is None else pred.keep + 2 + train_steps + 4
         for _ in range(warm):
             ts.step()
         ts.records = []
@@ -127,7 +241,10 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         its = [st.iterations for st in stats]
+        g0 = [st.guess_residual for st in stats if st.guess_residual is not None]
         out[name] = {"time_steps_per_s": round(steps / dt, 4), "mean_gmres_iters": float(np.mean(its)),
+                     "zero_iteration_steps": int(sum(i == 0 for i in its)),
+                     "mean_guess_residual": float(np.mean(g0)) if g0 else None,
                      "max_true_residual": float(max(st.residual for st in stats)),
                      "steps": steps, "untimed_steps": warm, "lu_refactor_every": ts.lu_every}
         if ts.record:
@@ -142,9 +259,12 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
         ts.records = []
         if pred is not None and pred.use_net:
             out[name]["predictor"] = pred.report
+    if cpu and "plain" in out:
+        out["cpu_baseline"] = oracle_step_estimate(
+            cfg, fibers, xs, ns, bodies or [], int(round(out["plain"]["mean_gmres_iters"])))
     if "plain" in out and "fast" in out:
         out["speedup_fast_vs_plain"] = round(out["fast"]["time_steps_per_s"] / out["plain"]["time_steps_per_s"], 3)
-    for m in ("extrapolated_fast", "learned_fast"):
+    for m in ("extrapolated_fast", "corrected_fast", "learned_fast"):
         if m in out:
             out[f"speedup_{m}_vs_plain"] = round(
                 out[m]["time_steps_per_s"] / out["plain"]["time_steps_per_s"], 3)
@@ -154,9 +274,9 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
 def small_leg(steps=30, cpu=True):
     """The small workload (32 fibers x 32 nodes, 1024-node periphery): GPU
     time-steps/s, plain FP64 and fast with the extrapolated guess, beside one
-    step of the oracle's FP64 restatement on the host (NumPy/SciPy: assembly,
-    per-fiber and shell LU, GMRES; the one-time shell matrix and its LU are
-    part of the oracle's set-up and are included)."""
+    step of the oracle's FP64 restatement on the host (NumPy/SciPy: per-step
+    assembly, per-fiber LU, GMRES; the one-time shell matrix and its LU are
+    set up once outside the timed step, as on the GPU)."""
     import torch
     from fiber3d_amd import configs
     from fiber3d_amd.learned import LearnedGuess
@@ -189,20 +309,31 @@ def small_leg(steps=30, cpu=True):
                      "max_residual": float(max(st.residual for st in stats))}
     if cpu:
         from oracle import tstep as OT
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        host = host_cores()
+        kw = dict(eta=cfg.eta, dt=cfg.dt, E=spec.E, epsilon=spec.epsilon,
+                  lengths=np.full(len(fibers), spec.length))
+        # one-time set-up (shell subtraction vectors, dense shell matrix and
+        # its LU: tstep.py:278-356) outside the timed step, as on the GPU
         t0 = time.perf_counter()
-        ora = OT.HydroSystem(fibers, (xs, ns, ws), eta=cfg.eta, dt=cfg.dt, E=spec.E,
-                             epsilon=spec.epsilon, lengths=np.full(len(fibers), spec.length))
-        t1 = time.perf_counter()
-        xo, info, its = OT.gmres(ora.matvec, ora.rhs, ora.precond, tol=cfg.gmres_tol,
-                                 restart=cfg.restart, maxiter=cfg.maxiter)
-        t2 = time.perf_counter()
-        res = float(np.linalg.norm(ora.rhs - ora.matvec(xo)) / np.linalg.norm(ora.rhs))
-        out["cpu_baseline"] = {"value": round(1.0 / (t2 - t0), 4), "unit": "time-steps/s",
-                               "cores": threads, "kind": "port",
+        first = OT.HydroSystem(fibers, (xs, ns, ws), **kw)
+        t_init = time.perf_counter() - t0
+        cache = first.shell_state()
+        times, its, res = [], 0, 0.0
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ora = OT.HydroSystem(fibers, (xs, ns, ws), shell_cache=cache, **kw)
+            xo, info, its = OT.gmres(ora.matvec, ora.rhs, ora.precond, tol=cfg.gmres_tol,
+                                     restart=cfg.restart, maxiter=cfg.maxiter)
+            times.append(time.perf_counter() - t0)
+            res = float(np.linalg.norm(ora.rhs - ora.matvec(xo)) / np.linalg.norm(ora.rhs))
+        step = float(np.median(times))
+        out["cpu_baseline"] = {"value": round(1.0 / step, 4), "unit": "time-steps/s",
+                               "cores": host["threads"], "kind": "port", "host": host,
                                "sample": "one plain FP64 step of this workload by the oracle "
-                                         "(NumPy/SciPy restatement of the reference's step)",
-                               "setup_s": round(t1 - t0, 3), "solve_s": round(t2 - t1, 3),
+                                         "(NumPy/SciPy restatement of the reference's step: "
+                                         "per-step assembly, fiber LU, GMRES), median of 3; "
+                                         "the one-time shell set-up excluded",
+                               "one_time_setup_s": round(t_init, 3),
                                "gmres_iters": int(its), "residual": res}
     return out
 
@@ -340,16 +471,21 @@ def main():
     ap.add_argument("--system", type=int, default=4,
                     help="also time implicit steps of fiber-system workload 0, 3 or 4 "
                          "(-1: raw sweep only)")
-    ap.add_argument("--system-steps", type=int, default=20)
-    ap.add_argument("--train-steps", type=int, default=8,
+    ap.add_argument("--system-steps", type=int, default=50)
+    ap.add_argument("--train-steps", type=int, default=30,
                     help="untimed steps the learned predictor trains on (0: plain and fast only)")
+    ap.add_argument("--no-series", dest="series", action="store_false",
+                    help="skip the time series (workload 3: 1000 fibers x 32 nodes, 8000-node "
+                         "periphery; the learned predictor trains on its first steps)")
+    ap.add_argument("--series-steps", type=int, default=300)
+    ap.add_argument("--series-train", type=int, default=100)
     ap.add_argument("--no-lu", dest="lu", action="store_false",
                     help="skip the batched fiber LU (workload 2) and MLP GEMM legs")
     ap.add_argument("--no-small", dest="small", action="store_false",
                     help="skip the small-workload time steps (GPU and oracle on the host)")
     ap.add_argument("--shell-pc", default="auto", choices=["auto", "dense", "block"])
-    ap.add_argument("--graph", action="store_true",
-                    help="replay the Arnoldi operator as a recorded HIP graph")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="run the Arnoldi steps eagerly instead of replaying recorded HIP graphs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (calibrates the HBM "
                          "byte counters FETCH_SIZE / WRITE_SIZE against a known size)")
@@ -444,9 +580,14 @@ def main():
         line["mlp_gemm_bf16"] = mlp_leg()
     if args.small:
         line["fiber_system_small"] = small_leg(cpu=not args.no_cpu)
+    if args.series:
+        line["fiber_system_series"] = system_leg(3, args.series_steps, args.series_train,
+                                                 shell_pc=args.shell_pc, graph=args.graph,
+                                                 cpu=not args.no_cpu)
     if args.system >= 0:
         line["fiber_system"] = system_leg(args.system, args.system_steps, args.train_steps,
-                                          shell_pc=args.shell_pc, graph=args.graph)
+                                          shell_pc=args.shell_pc, graph=args.graph,
+                                          cpu=not args.no_cpu)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(x, f)
     print(json.dumps(line), flush=True)

@@ -73,12 +73,16 @@ SIGNATURES = {
     "f3d_fiber_assemble": (ctypes.c_int, [
         _i64, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
         _f64, _f64, _f64, _f64, _f64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "f3d_fiber_rhs": (ctypes.c_int, [
+        _i64, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
+        _f64, _f64, _f64, _f64, _f64, _c_p, _c_p]),
     "f3d_gmres_workspace_size": (_sz, [_i64, _i32]),
     "f3d_gmres_begin": (ctypes.c_int, [_i64, _i32, _c_p, _c_p, _c_p, _c_p]),
     "f3d_gmres_current": (ctypes.c_int, [_i64, _i32, _c_p, _c_p, _c_p, _c_p]),
     "f3d_gmres_arnoldi": (ctypes.c_int, [_i64, _i32, _c_p, _c_p, _c_p, _c_p]),
     "f3d_gmres_solve": (ctypes.c_int, [_i64, _i32, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "f3d_dense_gemv_batched": (ctypes.c_int, [_i64, _i32, _c_p, _c_p, _c_p, _c_p]),
+    "f3d_dense_gemv_batched_f32": (ctypes.c_int, [_i64, _i32, _c_p, _c_p, _c_p, _c_p]),
     "f3d_block_gemv_batched": (ctypes.c_int, [
         _i64, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _f64, _f64, _c_p]),
 }

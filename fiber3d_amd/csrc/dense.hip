@@ -725,6 +725,169 @@ lu_solve_kernel(const int* __restrict__ n_arr, const long* __restrict__ a_off,
 }
 
 // ---------------------------------------------------------------------------
+// Solve without refinement with the factor staged in LDS: the workgroup's
+// four waves copy the block's packed factor (n^2 entries, at most
+// kLdsSolveBytes) into LDS in 16-byte pieces, all of it in flight at once,
+// and one wave then runs both substitutions out of LDS, where a column costs
+// an LDS read instead of an HBM round trip.  Same arithmetic as
+// lu_solve_kernel with refine = 0 (FP64 substitutions in the same order).
+constexpr int kLdsSolveBytes = 64 * 1024;
+
+// Substitutions out of LDS.  The loops run over the 64-row slot ms of the
+// pivot row k as an unrolled constant, so every register of w is addressed
+// statically, and the columns of the next kLdsGroup pivots are read while the
+// current group is applied (LDS latency off the dependent chain).
+constexpr int kLdsGroup = 8;
+
+template <typename T>
+__device__ __forceinline__ void trsv_lu_lds(const T* __restrict__ sL, int n, int lane,
+                                            double (&w)[kSlots]) {
+  constexpr int G = kLdsGroup;
+  // forward: w_i -= L(i, k) w_k for i > k (rows i > k sit in slots >= ms)
+#pragma unroll
+  for (int ms = 0; ms < kSlots; ++ms) {
+    const int k0 = 64 * ms, kend = min(64 * ms + 64, n - 1);
+    if (k0 >= kend) continue;
+    T cur[G][kSlots], nxt[G][kSlots];
+    auto load = [&](int kg, T (&buf)[G][kSlots]) __attribute__((always_inline)) {
+#pragma unroll
+      for (int u = 0; u < G; ++u)
+#pragma unroll
+        for (int m = ms; m < kSlots; ++m) {
+          const int k = kg + u, i = 64 * m + lane;
+          buf[u][m] = (k < kend && i > k && i < n) ? sL[(long)k * n + i] : (T)0;
+        }
+    };
+    load(k0, cur);
+    for (int kg = k0; kg < kend; kg += G) {
+      if (kg + G < kend) load(kg + G, nxt);
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        if (kg + u < kend) {
+          const double wk = readlane_f64(w[ms], (kg + u) & 63);
+#pragma unroll
+          for (int m = ms; m < kSlots; ++m) w[m] = fma(-(double)cur[u][m], wk, w[m]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u)
+#pragma unroll
+        for (int m = ms; m < kSlots; ++m) cur[u][m] = nxt[u][m];
+    }
+  }
+  // backward: w_k /= U(k, k); w_i -= U(i, k) w_k for i < k (slots <= ms)
+#pragma unroll
+  for (int ms = kSlots - 1; ms >= 0; --ms) {
+    const int khi = min(64 * ms + 63, n - 1), klo = 64 * ms;
+    if (khi < klo) continue;
+    T cur[G][kSlots], nxt[G][kSlots];
+    double dg[G], dn[G];
+    auto load = [&](int kg, T (&buf)[G][kSlots], double (&d)[G]) __attribute__((always_inline)) {
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int k = kg - u;
+#pragma unroll
+        for (int m = 0; m <= ms; ++m) {
+          const int i = 64 * m + lane;
+          buf[u][m] = (k >= klo && i < k) ? sL[(long)k * n + i] : (T)0;
+        }
+        d[u] = k >= klo ? (double)sL[(long)k * n + k] : 1.0;
+      }
+    };
+    load(khi, cur, dg);
+    for (int kg = khi; kg >= klo; kg -= G) {
+      if (kg - G >= klo) load(kg - G, nxt, dn);
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int k = kg - u;
+        if (k >= klo) {
+          const double wk = readlane_f64(w[ms], k & 63) / dg[u];
+#pragma unroll
+          for (int m = 0; m <= ms; ++m) {
+            const int i = 64 * m + lane;
+            w[m] = (i == k) ? wk : ((i < k) ? fma(-(double)cur[u][m], wk, w[m]) : w[m]);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        dg[u] = dn[u];
+#pragma unroll
+        for (int m = 0; m <= ms; ++m) cur[u][m] = nxt[u][m];
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+lu_solve_lds_kernel(int nmax, const int* __restrict__ n_arr, const long* __restrict__ a_off,
+                    const long* __restrict__ x_off, const T* __restrict__ LU,
+                    const double* __restrict__ rs, const double* __restrict__ cs,
+                    const int* __restrict__ perm, const double* __restrict__ x,
+                    double* __restrict__ y, int* __restrict__ iters) {
+  extern __shared__ __align__(16) unsigned char lds_solve[];
+  T* __restrict__ sL = reinterpret_cast<T*>(lds_solve);
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63;
+  const int n = n_arr[b];
+  // the whole workgroup leaves together (n is uniform), before any barrier;
+  // the launch sized the LDS for nmax
+  if (n < 1 || n > nmax) {
+    if (iters && t == 0) iters[b] = -1;
+    return;
+  }
+  const T* __restrict__ L = LU + a_off[b];
+  const long nn = (long)n * n;
+  if ((reinterpret_cast<uintptr_t>(L) & 15) == 0) {
+    constexpr int per = 16 / (int)sizeof(T);
+    const long nv = nn / per;
+    const uint4* __restrict__ src = reinterpret_cast<const uint4*>(L);
+    uint4* __restrict__ dst = reinterpret_cast<uint4*>(sL);
+    long e0 = t;
+    for (; e0 + 256 * 7 < nv; e0 += 256 * 8) {
+      uint4 v0 = src[e0], v1 = src[e0 + 256], v2 = src[e0 + 512], v3 = src[e0 + 768];
+      uint4 v4 = src[e0 + 1024], v5 = src[e0 + 1280], v6 = src[e0 + 1536], v7 = src[e0 + 1792];
+      dst[e0] = v0;
+      dst[e0 + 256] = v1;
+      dst[e0 + 512] = v2;
+      dst[e0 + 768] = v3;
+      dst[e0 + 1024] = v4;
+      dst[e0 + 1280] = v5;
+      dst[e0 + 1536] = v6;
+      dst[e0 + 1792] = v7;
+    }
+    for (; e0 < nv; e0 += 256) dst[e0] = src[e0];
+    for (long e = nv * per + t; e < nn; e += 256) sL[e] = L[e];
+  } else {
+    for (long e = t; e < nn; e += 256) sL[e] = L[e];
+  }
+  __syncthreads();
+  if (t >= 64) return;
+
+  const long xo = x_off[b];
+  const int* __restrict__ pm = perm + xo;
+  double w[kSlots];
+  int src_lane[kSlots], src_slot[kSlots];
+#pragma unroll
+  for (int m = 0; m < kSlots; ++m) {
+    const int i = 64 * m + lane;
+    const int sidx = i < n ? pm[i] : 0;
+    src_lane[m] = sidx & 63;
+    src_slot[m] = sidx >> 6;
+    const double rsv = (rs && i < n) ? rs[xo + i] : 1.0;
+    w[m] = (i < n ? x[xo + i] : 0.0) * rsv;  // still in A's row order
+  }
+  permute_rows(w, src_lane, src_slot, n, lane);
+  trsv_lu_lds<T>(sL, n, lane, w);
+#pragma unroll
+  for (int m = 0; m < kSlots; ++m) {
+    const int i = 64 * m + lane;
+    if (i < n) y[xo + i] = w[m] * ((cs && i < n) ? cs[xo + i] : 1.0);
+  }
+  if (iters && lane == 0) iters[b] = 0;
+}
+
+// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64)
 block_gemv_kernel(const int* __restrict__ m_arr, const int* __restrict__ k_arr,
                   const long* __restrict__ m_off, const long* __restrict__ x_off,
@@ -775,50 +938,105 @@ block_gemv_kernel(const int* __restrict__ m_arr, const int* __restrict__ k_arr,
 }
 
 // ---------------------------------------------------------------------------
-// y_b = M_b x_b for `count` dense ROW-major m x m FP64 matrices, one wave per
-// row (four rows per 256-thread workgroup).  A wave reads its row in 1 KB
-// coalesced steps (16 bytes per lane when `pairs` is set: m even, M and x
-// 16-byte aligned), accumulates in two FP64 chains and reduces across the
-// wave in a fixed order, so the result is deterministic.  Even one matrix of
-// a few thousand rows puts thousands of waves in flight, which an HBM-bound
-// sweep over a large inverse needs; the column-major row-block GEMV above
-// runs one wave per 256-row block.
+// y_b = M_b x_b for `count` dense ROW-major m x m matrices (FP64, or FP32
+// entries with FP64 products and sums).  One wave computes kGemvRows rows of
+// one matrix (fewer at the matrix's end), four waves per 256-thread
+// workgroup: each 16-byte piece of x it loads serves all its rows, which
+// keeps the x reads (from L2) at a quarter of the matrix bytes.  A wave reads
+// each row in 1 KB coalesced steps (16 bytes per lane when `vec` is set: m a
+// multiple of 16 / sizeof(TM), M and x 16-byte aligned), accumulates each row
+// in two FP64 chains and reduces across the wave in a fixed order, so the
+// result is deterministic.  Even one matrix of a few thousand rows puts
+// thousands of waves in flight, which an HBM-bound sweep over a large inverse
+// needs; the column-major row-block GEMV above runs one wave per 256-row
+// block.
+constexpr int kGemvRows = 4;
+
+__device__ __forceinline__ void gemv_fma16(const double2 a, const double2* __restrict__ v,
+                                           double& acc0, double& acc1) {
+  acc0 = fma(a.x, v[0].x, acc0);
+  acc1 = fma(a.y, v[0].y, acc1);
+}
+
+__device__ __forceinline__ void gemv_fma16(const float4 a, const double2* __restrict__ v,
+                                           double& acc0, double& acc1) {
+  acc0 = fma((double)a.x, v[0].x, acc0);
+  acc1 = fma((double)a.y, v[0].y, acc1);
+  acc0 = fma((double)a.z, v[1].x, acc0);
+  acc1 = fma((double)a.w, v[1].y, acc1);
+}
+
+template <typename TM> struct Gemv16;
+template <> struct Gemv16<double> { using V = double2; };
+template <> struct Gemv16<float> { using V = float4; };
+
+template <typename TM>
 __global__ void __launch_bounds__(256)
-dense_gemv_kernel(int64_t rows_total, int32_t m, int pairs, const double* __restrict__ M,
-                  const double* __restrict__ x, double* __restrict__ y) {
+dense_gemv_kernel(int64_t waves_total, int32_t m, int32_t groups, int vec,
+                  const TM* __restrict__ M, const double* __restrict__ x,
+                  double* __restrict__ y) {
+  constexpr int R = kGemvRows;
   const int lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= rows_total) return;
-  const double* __restrict__ Mr = M + row * (int64_t)m;
-  const double* __restrict__ xb = x + (row / m) * (int64_t)m;
-  double acc0 = 0.0, acc1 = 0.0;
-  if (pairs) {
-    const double2* __restrict__ M2 = reinterpret_cast<const double2*>(Mr);
+  const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= waves_total) return;
+  const int64_t b = w / groups;
+  const int r0 = (int)(w - b * groups) * R;
+  const int nr = min(R, m - r0);
+  const double* __restrict__ xb = x + b * (int64_t)m;
+  // rows past the matrix's end repeat its last row (loaded again from
+  // cache, never stored)
+  const TM* __restrict__ Mr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) Mr[r] = M + (b * (int64_t)m + r0 + min(r, nr - 1)) * (int64_t)m;
+  double acc0[R], acc1[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc0[r] = acc1[r] = 0.0;
+  if (vec) {
+    using V = typename Gemv16<TM>::V;
+    constexpr int per = (int)(16 / sizeof(TM));  // matrix entries per 16 bytes
+    constexpr int xv = per / 2;                  // double2 of x per 16 bytes of M
+    const int nv = m / per;
     const double2* __restrict__ x2 = reinterpret_cast<const double2*>(xb);
-    const int np = m >> 1;
     int p = lane;
-    for (; p + 192 < np; p += 256) {
-      const double2 a0 = M2[p], a1 = M2[p + 64], a2 = M2[p + 128], a3 = M2[p + 192];
-      const double2 v0 = x2[p], v1 = x2[p + 64], v2 = x2[p + 128], v3 = x2[p + 192];
-      acc0 = fma(a0.x, v0.x, acc0);
-      acc1 = fma(a0.y, v0.y, acc1);
-      acc0 = fma(a1.x, v1.x, acc0);
-      acc1 = fma(a1.y, v1.y, acc1);
-      acc0 = fma(a2.x, v2.x, acc0);
-      acc1 = fma(a2.y, v2.y, acc1);
-      acc0 = fma(a3.x, v3.x, acc0);
-      acc1 = fma(a3.y, v3.y, acc1);
+    for (; p + 64 < nv; p += 128) {
+      V a[R][2];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        a[r][0] = reinterpret_cast<const V*>(Mr[r])[p];
+        a[r][1] = reinterpret_cast<const V*>(Mr[r])[p + 64];
+      }
+      double2 v0[xv], v1[xv];
+#pragma unroll
+      for (int q = 0; q < xv; ++q) {
+        v0[q] = x2[p * xv + q];
+        v1[q] = x2[(p + 64) * xv + q];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        gemv_fma16(a[r][0], v0, acc0[r], acc1[r]);
+        gemv_fma16(a[r][1], v1, acc0[r], acc1[r]);
+      }
     }
-    for (; p < np; p += 64) {
-      const double2 a = M2[p], v = x2[p];
-      acc0 = fma(a.x, v.x, acc0);
-      acc1 = fma(a.y, v.y, acc1);
+    for (; p < nv; p += 64) {
+      double2 v0[xv];
+#pragma unroll
+      for (int q = 0; q < xv; ++q) v0[q] = x2[p * xv + q];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        gemv_fma16(reinterpret_cast<const V*>(Mr[r])[p], v0, acc0[r], acc1[r]);
     }
   } else {
-    for (int j = lane; j < m; j += 64) acc0 = fma(Mr[j], xb[j], acc0);
+    for (int j = lane; j < m; j += 64) {
+      const double xj = xb[j];
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc0[r] = fma((double)Mr[r][j], xj, acc0[r]);
+    }
   }
-  const double sum = wave_sum_f64(acc0 + acc1);
-  if (lane == 0) y[row] = sum;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const double sum = wave_sum_f64(acc0[r] + acc1[r]);
+    if (lane == 0 && r < nr) y[b * (int64_t)m + r0 + r] = sum;
+  }
 }
 
 static int num_cus() {
@@ -910,6 +1128,25 @@ extern "C" int f3d_lu_solve_batched(int precision, int64_t nb, int32_t nmax, con
   F3D_REQUIRE(n && a_off && x_off && perm && x && y, F3D_EINVAL, "lu_solve: NULL argument");
   F3D_REQUIRE(refine == 0 || A != nullptr, F3D_EINVAL, "lu_solve: refinement needs A");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  // without refinement, factors that fit the LDS budget are staged there
+  const size_t elem = precision == F3D_F64 ? sizeof(double) : sizeof(float);
+  const size_t lds = align_up((size_t)nmax * nmax * elem, 16);
+  // F3D_LU_SOLVE_STREAM=1 keeps the streaming kernel (for comparison)
+  const char* force_stream = getenv("F3D_LU_SOLVE_STREAM");
+  const bool staged = refine == 0 && lds <= (size_t)kLdsSolveBytes &&
+                      !(force_stream && *force_stream && *force_stream != '0');
+  if (staged && precision == F3D_F64) {
+    F3D_REQUIRE(lu64 != nullptr, F3D_EINVAL, "lu_solve: lu64 is NULL");
+    F3D_LAUNCH((lu_solve_lds_kernel<double>), dim3((unsigned)nb), dim3(256), lds, st, nmax,
+               n, (const long*)a_off, (const long*)x_off, lu64, rs, cs, perm, x, y, iters);
+    return F3D_OK;
+  }
+  if (staged) {
+    F3D_REQUIRE(lu32 && rs && cs, F3D_EINVAL, "lu_solve: lu32/rs/cs is NULL");
+    F3D_LAUNCH((lu_solve_lds_kernel<float>), dim3((unsigned)nb), dim3(256), lds, st, nmax,
+               n, (const long*)a_off, (const long*)x_off, lu32, rs, cs, perm, x, y, iters);
+    return F3D_OK;
+  }
   if (precision == F3D_F64) {
     F3D_REQUIRE(lu64 != nullptr, F3D_EINVAL, "lu_solve: lu64 is NULL");
     F3D_LAUNCH((lu_solve_kernel<double>), dim3((unsigned)nb), dim3(64), 0, st, n,
@@ -924,19 +1161,33 @@ extern "C" int f3d_lu_solve_batched(int precision, int64_t nb, int32_t nmax, con
   return F3D_OK;
 }
 
-extern "C" int f3d_dense_gemv_batched(int64_t count, int32_t m, const double* M,
-                                      const double* x, double* y, void* stream) {
+template <typename TM>
+static int dense_gemv_launch(int64_t count, int32_t m, const TM* M, const double* x, double* y,
+                             void* stream) {
   F3D_REQUIRE(count >= 0 && m >= 0, F3D_EINVAL, "dense_gemv: count=%lld m=%d",
               (long long)count, m);
   if (count == 0 || m == 0) return F3D_OK;
   F3D_REQUIRE(M && x && y, F3D_EINVAL, "dense_gemv: NULL argument");
-  const int64_t rows = count * (int64_t)m;
-  F3D_REQUIRE(rows / 4 < (1ll << 31), F3D_EINVAL, "dense_gemv: %lld rows", (long long)rows);
-  const int pairs = (m % 2 == 0) && (((uintptr_t)M | (uintptr_t)x) % 16 == 0);
+  const int32_t groups = (m + kGemvRows - 1) / kGemvRows;
+  const int64_t waves = count * (int64_t)groups;
+  F3D_REQUIRE((waves + 3) / 4 < (1ll << 31), F3D_EINVAL, "dense_gemv: %lld x %d rows",
+              (long long)count, m);
+  const int per = (int)(16 / sizeof(TM));
+  const int vec = (m % per == 0) && (((uintptr_t)M | (uintptr_t)x) % 16 == 0);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  F3D_LAUNCH(dense_gemv_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st, rows, m,
-             pairs, M, x, y);
+  F3D_LAUNCH((dense_gemv_kernel<TM>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st,
+             waves, m, groups, vec, M, x, y);
   return F3D_OK;
+}
+
+extern "C" int f3d_dense_gemv_batched(int64_t count, int32_t m, const double* M,
+                                      const double* x, double* y, void* stream) {
+  return dense_gemv_launch<double>(count, m, M, x, y, stream);
+}
+
+extern "C" int f3d_dense_gemv_batched_f32(int64_t count, int32_t m, const float* M,
+                                          const double* x, double* y, void* stream) {
+  return dense_gemv_launch<float>(count, m, M, x, y, stream);
 }
 
 extern "C" int f3d_block_gemv_batched(int64_t nb, int32_t mmax, const int32_t* m,

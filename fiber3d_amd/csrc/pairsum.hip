@@ -428,6 +428,12 @@ struct ShellEpi {
   const double* sigma_part;  // kPackBlocks partial sums of n.q
 };
 
+// Targets per block of reduce_kernel: its four waves split the source
+// chunks p = w, w + 4, ... of the same 64 targets (4x the waves of one
+// target per lane, so enough loads are in flight to cover HBM latency at
+// 40k targets), then wave 0 adds the four sums in a fixed order.
+constexpr int kRedTargets = 64;
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 reduce_kernel(const T* __restrict__ part_hi, const T* __restrict__ part_lo, int P, int nt,
@@ -443,19 +449,31 @@ reduce_kernel(const T* __restrict__ part_hi, const T* __restrict__ part_lo, int 
     __syncthreads();
     sigma = red[0] + red[1] + red[2] + red[3];
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nt) return;
+  __shared__ double sums[3][3][kRedTargets];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = blockIdx.x * kRedTargets + lane;
+  double a[3] = {0.0, 0.0, 0.0};
+  if (i < nt) {
+#pragma unroll 2
+    for (int p = wv; p < P; p += 4) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const size_t idx = ((size_t)p * 3 + c) * nt + i;
+        a[c] += (double)part_hi[idx];
+        if (sizeof(T) == 4) a[c] += (double)part_lo[idx];
+      }
+    }
+  }
+  if (wv > 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sums[wv - 1][c][lane] = a[c];
+  }
+  __syncthreads();
+  if (wv != 0 || i >= nt) return;
   double u[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double acc = 0.0;
-    for (int p = 0; p < P; ++p) {
-      const size_t idx = ((size_t)p * 3 + c) * nt + i;
-      acc += (double)part_hi[idx];
-      if (sizeof(T) == 4) acc += (double)part_lo[idx];
-    }
-    u[c] = factor * acc;
-  }
+  for (int c = 0; c < 3; ++c)
+    u[c] = factor * (((a[c] + sums[0][c][lane]) + sums[1][c][lane]) + sums[2][c][lane]);
   const long b = (long)i * o_sp;
   if (epi.q) {
     const double wi = epi.w[i];
@@ -634,7 +652,7 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
     epi = *epi_in;
     epi.sigma_part = sigma_part;
   }
-  F3D_LAUNCH((reduce_kernel<T>), dim3((unsigned)ceil_div(nt, 256)), dim3(256), 0, st, part_hi,
+  F3D_LAUNCH((reduce_kernel<T>), dim3((unsigned)ceil_div(nt, (int64_t)kRedTargets)), dim3(256), 0, st, part_hi,
              part_lo, pl.P, (int)nt, factor, (const Geo*)geo, Degree<KIND>::value, out,
              (long)o_sp, (long)o_sc, (flags & F3D_ACCUMULATE) ? 1 : 0, epi);
   return F3D_OK;
@@ -670,7 +688,7 @@ static int pairsum_impl(int kernel, int precision, uint32_t flags, int64_t ns, c
   if (ns == 0 && epi == nullptr) {
     if (flags & F3D_ACCUMULATE) return F3D_OK;
     // out = 0 on the caller's layout: reuse the reduce kernel with P = 0.
-    F3D_LAUNCH((reduce_kernel<double>), dim3((unsigned)ceil_div(nt, 256)), dim3(256), 0, st,
+    F3D_LAUNCH((reduce_kernel<double>), dim3((unsigned)ceil_div(nt, (int64_t)kRedTargets)), dim3(256), 0, st,
                (const double*)nullptr, (const double*)nullptr, 0, (int)nt, 0.0,
                (const Geo*)nullptr, 1, out, (long)o_sp, (long)o_sc, 0, ShellEpi{});
     return F3D_OK;

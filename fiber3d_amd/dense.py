@@ -146,29 +146,32 @@ class BlockGemv:
 
 
 class DenseGemv:
-    """y = M x for a stack of dense square FP64 matrices through
-    f3d_dense_gemv_batched (row-major, one wave per row).  Unlike a BLAS GEMV
-    this allocates nothing per call, so it can sit inside a recorded HIP
+    """y = M x for a stack of dense square matrices through
+    f3d_dense_gemv_batched (FP64 entries) or f3d_dense_gemv_batched_f32 (FP32
+    entries, FP64 arithmetic): row-major, one wave per row.  Unlike a BLAS
+    GEMV this allocates nothing per call, so it can sit inside a recorded HIP
     graph."""
 
     def __init__(self, mats):
-        """mats: (count, m, m) or (m, m) FP64 CUDA tensor."""
+        """mats: (count, m, m) or (m, m) FP64 or FP32 CUDA tensor."""
         if mats.dim() == 2:
             mats = mats[None]
         count, m, m2 = mats.shape
-        assert m == m2 and mats.dtype == torch.float64
+        assert m == m2 and mats.dtype in (torch.float64, torch.float32)
         self.count, self.m = count, m
         self.M = mats.contiguous()
+        self._fn = ("f3d_dense_gemv_batched" if mats.dtype == torch.float64
+                    else "f3d_dense_gemv_batched_f32")
 
     def __call__(self, x, y):
         """x, y: contiguous (count * m,) FP64 tensors (y must not alias x)."""
         assert x.numel() == y.numel() == self.count * self.m
+        assert x.dtype == y.dtype == torch.float64
         assert x.is_contiguous() and y.is_contiguous() and x.data_ptr() != y.data_ptr()
-        rc = L.load().f3d_dense_gemv_batched(self.count, self.m, L.ptr(self.M), L.ptr(x),
-                                             L.ptr(y), _stream())
-        L.check(rc, "f3d_dense_gemv_batched")
+        rc = getattr(L.load(), self._fn)(self.count, self.m, L.ptr(self.M), L.ptr(x),
+                                         L.ptr(y), _stream())
+        L.check(rc, self._fn)
         return y
-
 
 
 def build_block_diagonal_lu_preconditioner_fiber(fibers, A_fibers_blocks):

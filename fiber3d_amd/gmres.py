@@ -75,6 +75,8 @@ class GmresResult:
     residual_kind: str = "fp64"  # "fp64": b - A x recomputed; "update": FP64-anchored update
     residual_bound: float = 0.0  # bound on the true relative residual
     inner_err: float = None      # measured ||(A_inner - A) z|| / ||A_inner z|| (max over cycles)
+    resid: torch.Tensor = None   # the residual vector the closing decision used (b - A x,
+                                 # or the FP64-anchored update)
 
 
 class GmresContext:
@@ -258,7 +260,7 @@ def _gmres_host(matvec, b, precond, tol, restart, maxiter, x0, inner, callback, 
         hist.append(beta / bnorm)
         if beta <= tol * bnorm:
             return GmresResult(x, 0, total, restarts, beta / bnorm, guess_used, hist,
-                               inner_cycles, "fp64", beta / bnorm, measured)
+                               inner_cycles, "fp64", beta / bnorm, measured, r)
         V[0] = r / beta
         H = np.zeros((m + 1, m))
         Hraw = np.zeros((m + 1, m))  # unrotated Hessenberg: A P^-1 V_k = V_{k+1} Hraw
@@ -319,7 +321,8 @@ def _gmres_host(matvec, b, precond, tol, restart, maxiter, x0, inner, callback, 
             if update_residual and est <= t_it and bound <= tol * bnorm:
                 hist.append(rn_upd / bnorm)
                 return GmresResult(x, 0, total, restarts, rn_upd / bnorm, guess_used, hist,
-                                   inner_cycles + 1, "update", bound / bnorm, measured)
+                                   inner_cycles + 1, "update", bound / bnorm, measured,
+                                   r_upd.clone())
         r = b - matvec(x)
         if r_upd is not None and hyn > 0.0:
             # r_old - r_upd is the correction's image under the inner
@@ -334,7 +337,7 @@ def _gmres_host(matvec, b, precond, tol, restart, maxiter, x0, inner, callback, 
                 op = matvec
     rn = float(torch.linalg.norm(r)) / bnorm
     return GmresResult(x, 0 if rn <= tol else total, total, restarts, rn, guess_used, hist,
-                       inner_cycles, "fp64", rn, measured)
+                       inner_cycles, "fp64", rn, measured, r)
 
 
 def _gmres_device(matvec, b, precond, tol, restart, maxiter, x0, inner, callback, fused_inner,
@@ -354,20 +357,27 @@ def _gmres_device(matvec, b, precond, tol, restart, maxiter, x0, inner, callback
     inner_cycles = 0
     measured = None
     n = b.numel()
-    bnorm = float(torch.linalg.norm(b))
     hist = []
+    guess_used = False
+    beta0 = None
+    if x0 is not None:
+        # |b| and the guess's residual |b - A x0| in one read: a non-finite
+        # guess gives a non-finite residual and is dropped like a poor one
+        r0 = b - matvec(x0)
+        bnorm, r0n = torch.stack([torch.linalg.norm(b), torch.linalg.norm(r0)]).tolist()
+    else:
+        bnorm, r0n = float(torch.linalg.norm(b)), float("nan")
     if bnorm == 0.0:
         return GmresResult(torch.zeros_like(b), 0, 0, 0, 0.0, False, hist, 0)
-    guess_used = False
-    x = torch.zeros_like(b)
-    r = b.clone()
-    if x0 is not None:
-        if bool(torch.isfinite(x0).all()):
-            r0 = b - matvec(x0)
-            if float(torch.linalg.norm(r0)) < bnorm:
-                x = x0.clone()
-                r = r0
-                guess_used = True
+    if math.isfinite(r0n) and r0n < bnorm:
+        x = x0.clone()
+        r = r0
+        guess_used = True
+        beta0 = r0n
+    else:
+        x = torch.zeros_like(b)
+        r = b.clone()
+        beta0 = bnorm
     m = min(restart, n)
     if ctx is None or ctx.n != n or ctx.m != m:
         ctx = GmresContext(n, m, b.device)
@@ -375,11 +385,12 @@ def _gmres_device(matvec, b, precond, tol, restart, maxiter, x0, inner, callback
     total = restarts = 0
     ahead = max(0, int(expect))
     for _cycle in range(maxiter):
-        beta = float(torch.linalg.norm(r))
+        beta = beta0 if beta0 is not None else float(torch.linalg.norm(r))
+        beta0 = None
         hist.append(beta / bnorm)
         if beta <= tol * bnorm:
             return GmresResult(x, 0, total, restarts, beta / bnorm, guess_used, hist,
-                               inner_cycles, "fp64", beta / bnorm, measured)
+                               inner_cycles, "fp64", beta / bnorm, measured, r)
         ctx.begin(r)
         fused = fused_matvec if op is matvec else fused_inner
         if fused is not None:
@@ -423,7 +434,8 @@ def _gmres_device(matvec, b, precond, tol, restart, maxiter, x0, inner, callback
             if update_residual and est <= t_it and bound <= tol * bnorm:
                 hist.append(rn_upd / bnorm)
                 return GmresResult(x, 0, total, restarts, rn_upd / bnorm, guess_used, hist,
-                                   inner_cycles + 1, "update", bound / bnorm, measured)
+                                   inner_cycles + 1, "update", bound / bnorm, measured,
+                                   r_upd.clone())
         r = b - matvec(x)
         if r_upd is not None and hyn > 0.0:
             # the update's whole error, as in _gmres_host
@@ -435,4 +447,4 @@ def _gmres_device(matvec, b, precond, tol, restart, maxiter, x0, inner, callback
                 op = matvec
     rn = float(torch.linalg.norm(r)) / bnorm
     return GmresResult(x, 0 if rn <= tol else total, total, restarts, rn, guess_used, hist,
-                       inner_cycles, "fp64", rn, measured)
+                       inner_cycles, "fp64", rn, measured, r)

@@ -5,15 +5,15 @@ python/tstep/tstep.py:1209-1216).  The plain zero-guess solve stays the
 fallback: fiber3d_amd.gmres drops a guess that is not finite or whose residual
 exceeds ||b||, and the FP64 residual alone decides convergence.
 
-Network: 3 layers, width 256, ReLU, applied independently to every fiber node
-(predicting the correction of the extrapolated position in a local frame) and
-every shell node (predicting coefficients of the density correction in a basis
-built from local vectors).  Forward and backward products run on MFMA in BF16
-with FP32 accumulation (libf3d f3d_gemm_bf16); master weights and the Adam
-update are FP32.
+Network: 3 layers, width 256, ReLU, applied independently to every fiber
+node, predicting the correction of the extrapolated position (in a local
+frame) and tension from the local geometry, the history and the solver's
+residual signal.  Forward and backward products run on MFMA in BF16 with FP32
+accumulation (libf3d f3d_gemm_bf16); master weights and the Adam update are
+FP32.
 
 Guess assembly per step: the previous solutions extrapolated (positions by a
-least-squares quadratic in time, the rest linearly), corrected by the nets
+least-squares quadratic in time, the rest linearly), corrected by the net
 (see LearnedGuess).
 """
 import numpy as np
@@ -157,56 +157,153 @@ class LearnedGuess:
 
     Baseline guess: every component extrapolated linearly from the last two
     solutions, except the fiber positions X, which come from the least-squares
-    polynomial of degree fit[1] through the last fit[0] positions.  Nearly all
-    of the guess's residual is in X: putting the converged X into the guess
-    leaves no GMRES iteration to do, while the converged tensions or shell
-    densities barely change the count.  The linear extrapolation of X
-    carries a second difference dt^2 X'' that stays nearly constant from step
-    to step.  A quadratic through three points removes it but multiplies the
-    solves' own errors by up to 7.  The least-squares quadratic through six
-    points removes it and multiplies them by at most 3.6.
+    polynomial of degree fit[1] through the last fit[0] positions.  The linear
+    extrapolation of X carries a second difference dt^2 X'' that stays nearly
+    constant from step to step; the least-squares quadratic through six
+    points removes it and multiplies the solves' own errors by at most 3.6.
 
-    Two MLPs correct the baseline.  One per fiber node predicts the position
-    correction X_n - X_fit as its components in an orthonormal local frame
-    (tangent, the part of the last displacement normal to it, their cross
-    product).  One per shell node predicts coefficients c_k of the density
-    correction q_n - q_ext = sum_k c_k u_k in a local basis u = (dq, d2q,
-    n (n . dq)) built from the density history.  Inputs are rotation-invariant
-    scalars of the local geometry, the explicit flow and the solution history.
-    The last layer starts at zero, so an untrained net reproduces the
-    baseline.  After training, each net is kept only if it beats the baseline
-    on held-out steps (validation MSE below `gate` times the baseline's).
+    Residual correction.  After each solve the stepper hands the predictor
+    c = P^-1 r, the preconditioner applied to the residual vector the solve
+    closed with (|r| <= tol |b|); x + c approximates the exact solution better
+    than x by about the preconditioner's quality.  Solutions are accepted
+    just below the tolerance, and extrapolating them passes their errors on:
+    the plain guess's residual sits near the tolerance.  correct=True
+    extrapolates the corrected history x_k + c_k instead.
+
+    The net (use_net=True): one MLP per fiber node predicts the correction
+    of the baseline guess (the extrapolation of the corrected history when
+    the stepper hands over corrections, else the plain extrapolation).  Its
+    target is the corrected solution x + P^-1 r minus that guess: the
+    position part in an orthonormal local frame (tangent, the part of the
+    last displacement normal to it, their cross product) and the tension
+    part.  Its features are rotation-invariant: the local geometry, the
+    explicit flow, the solution history and the solver's residual signal
+    (the corrected-minus-plain extrapolation and the last correction, in the
+    same frame).  Its correction is projected, fiber by
+    fiber, onto the first `modes` Legendre polynomials of the fiber
+    parameter: the fiber operator's fourth derivatives amplify node-to-node
+    noise by orders of magnitude, so only smooth corrections help.  Its last
+    layer starts at zero, so an untrained net reproduces the baseline.  After
+    training it must beat the baseline twice on held-out data: its validation
+    MSE below `gate` times the baseline's on the last val_frac of the
+    training steps, then its guesses' residuals (FP32-compensated products)
+    below `gate` times the baseline guesses' over the next `shadow_steps`
+    steps, during which the baseline still drives the solves.  Other
+    components keep the plain extrapolation.
     """
 
     def __init__(self, train_steps=100, epochs=200, lr=1e-3, batch=8192, seed=0,
-                 use_net=True, gate=0.9, val_frac=0.2, fit=(6, 2)):
+                 use_net=True, gate=0.9, val_frac=0.2, fit=(6, 2), correct=False,
+                 max_samples=None, modes=8, shadow_steps=4):
         self.fit = (int(fit[0]), int(fit[1]))
         lsq_weights(*self.fit)  # validates the shape
         self.keep = max(3, self.fit[0])
         self.train_steps = train_steps
         self.epochs, self.lr, self.batch, self.seed = epochs, lr, batch, seed
         self.use_net = use_net
+        self.correct = bool(correct) and not use_net
         self.gate, self.val_frac = gate, val_frac
+        # training samples kept per step (a seeded random subset of the fiber
+        # nodes when there are more)
+        self.max_samples = max_samples
+        self._rng = torch.Generator().manual_seed(seed)
         self.samples_f = []
-        self.samples_s = []
-        self.prev = []   # last `keep` solutions, oldest first
+        # history ring buffers (keep x n): solutions and corrected solutions
+        # x + P^-1 r; _rows lists the rows in use, oldest first
+        self._H = self._Hc = None
+        self._rows = []
+        self._cor_ok = []  # per row in use: was a correction given
+        self._last_corr = None
+        self._wcache = {}
+        self._mask_x = None
         self.times = []  # their times
-        self.net_f = self.net_s = None
-        self.enabled_f = self.enabled_s = False
+        self.net_f = None
+        self.enabled_f = False
+        self.modes = modes
+        self.shadow_steps = shadow_steps
+        self.shadow_left = 0
+        self.shadow = []
         self.trained = False
         self.report = {}
 
-    # ---- features ---------------------------------------------------------
-    def _hist(self, ts, idx):
-        """Any gathered component at the last three solutions (newest first)."""
-        return [self.prev[-1 - k][idx] for k in range(3)]
+    @property
+    def wants_correction(self):
+        """Whether the stepper should pass P^-1 r to observe() (one
+        preconditioner apply per step, used by the corrected and net modes)."""
+        return self.correct or self.use_net
 
-    def _fiber_terms(self, ts):
-        """Per fiber node: 15 rotation-invariant features and the orthonormal
+    # ---- history and extrapolation -----------------------------------------
+    @property
+    def prev(self):
+        """The last `keep` solutions, oldest first (views of the ring)."""
+        return [] if self._H is None else [self._H[r] for r in self._rows]
+
+    def _push(self, sol, corr):
+        n = sol.numel()
+        if self._H is None or self._H.shape[1] != n:
+            self._H = torch.zeros((self.keep, n), dtype=sol.dtype, device=sol.device)
+            self._Hc = torch.zeros_like(self._H)
+            self._rows, self._cor_ok = [], []
+        pos = (self._rows[-1] + 1) % self.keep if self._rows else 0
+        self._H[pos].copy_(sol)
+        if corr is not None:
+            torch.add(sol, corr, out=self._Hc[pos])
+        self._rows = (self._rows + [pos])[-self.keep:]
+        self._cor_ok = (self._cor_ok + [corr is not None])[-self.keep:]
+        self._last_corr = corr.clone() if (corr is not None and self.use_net) else None
+
+    def _weights(self, W):
+        """Device copy of the (2, keep) weight matrix, cached by value (the
+        weights repeat while the step size does)."""
+        key = W.tobytes()
+        t = self._wcache.get(key)
+        if t is None:
+            if len(self._wcache) > 32:
+                self._wcache.clear()
+            t = torch.from_numpy(W).to(self._H.device)
+            self._wcache[key] = t
+        return t
+
+    def _extrapolated(self, ts, corrected=False):
+        """Linear extrapolation in time of every component of the history
+        (corrected: of x + P^-1 r) to the coming solution's time (2 p1 - p2 at
+        equal steps); fiber positions from the least-squares polynomial fit
+        once fit[0] solutions exist.  The weights use the solutions' actual
+        times, so steps of changing size are extrapolated correctly.  Both
+        combinations are one (2 x keep) x (keep x n) product over the ring."""
+        t_next = ts.time + ts.sys.dt
+        t1, t2 = self.times[-1], self.times[-2]
+        a = (t_next - t1) / (t1 - t2)
+        rows = self._rows
+        W = np.zeros((2, self.keep))
+        W[0, rows[-1]] = 1.0 + a
+        W[0, rows[-2]] = -a
+        npts, deg = self.fit
+        full = len(rows) >= npts
+        if full:
+            w = lsq_weights_at(self.times[::-1][:npts], t_next, deg)
+            for k, wk in enumerate(w):
+                W[1, rows[-1 - k]] = wk
+        R = self._weights(W) @ (self._Hc if corrected else self._H)
+        if not full:
+            return R[0]
+        if self._mask_x is None or self._mask_x.numel() != R.shape[1]:
+            m = torch.zeros(R.shape[1], dtype=torch.bool, device=R.device)
+            m[ts.idx_x.reshape(-1)] = True
+            self._mask_x = m
+        return torch.where(self._mask_x, R[1], R[0])
+
+    def _have_corrected(self):
+        return bool(self._cor_ok) and all(self._cor_ok)
+
+    # ---- features -----------------------------------------------------------
+    def _fiber_terms(self, ts, x_ext, x_cor):
+        """Per fiber node: 23 rotation-invariant features and the orthonormal
         frame (t, e2, e3) the position correction is expressed in."""
         S = ts.sys
-        X1, X2, X3 = (p.view(-1, 3) for p in self._hist(ts, ts.idx_x))
-        T1, T2, _ = self._hist(ts, ts.idx_t)
+        ix = ts.idx_x
+        X1, X2, X3 = (self.prev[-1 - k][ix] for k in range(3))
+        T1, T2 = self.prev[-1][ts.idx_t], self.prev[-2][ts.idx_t]
         d1 = X1 - X2
         d2 = X1 - 2.0 * X2 + X3
         t = S.xs / torch.linalg.norm(S.xs, dim=1, keepdim=True).clamp_min(1e-300)
@@ -229,94 +326,108 @@ class LearnedGuess:
             alpha[node.reshape(-1)] = S.buckets[N]["alpha"][None, :].expand(B, N).reshape(-1)
             xss[node.reshape(-1)] = xss_k.reshape(-1, 3)
         c1, c2, cv, cs = comp(d1), comp(d2), comp(S.v_explicit), comp(xss)
+        # residual signal: corrected minus plain extrapolation, and the last
+        # correction (zero where no correction was given)
+        sig = (x_cor - x_ext) if x_cor is not None else torch.zeros_like(x_ext)
+        lc = self._last_corr if self._last_corr is not None else torch.zeros_like(x_ext)
         feats = torch.cat([alpha[:, None], alpha.abs()[:, None], c1[:, :2], c2, cv, cs,
-                           T1[:, None], (T1 - T2)[:, None]], dim=1)
+                           T1[:, None], (T1 - T2)[:, None],
+                           comp(sig[ix]), sig[ts.idx_t][:, None],
+                           comp(lc[ix]), lc[ts.idx_t][:, None]], dim=1)
         return feats, frame
 
-    def _shell_terms(self, ts):
-        S = ts.sys
-        q1, q2, q3 = (p[:3 * S.nper].view(S.nper, 3) for p in self._hist(ts, slice(None)))
-        dq = q1 - q2
-        d2q = dq - (q2 - q3)
-        n = S.shell.nrm
-        v = -S.rhs[:3 * S.nper].view(S.nper, 3)  # explicit fiber flow on the shell
-        u = torch.stack([dq, d2q, n * (n * dq).sum(-1, keepdim=True)], dim=1)  # (nper, 3, 3)
-        a, b = dq.norm(dim=-1), d2q.norm(dim=-1)
-        inv = torch.stack([a, b, (n * dq).sum(-1), (n * d2q).sum(-1),
-                           (dq * d2q).sum(-1) / (a * b + 1e-300), v.norm(dim=-1),
-                           (n * v).sum(-1)], dim=-1)
-        return u, inv
-
-    def _extrapolated(self, ts):
-        """Linear extrapolation in time of every component to the time of the
-        coming solution (2 p1 - p2 at equal steps); fiber positions from the
-        least-squares polynomial fit once fit[0] solutions exist.  The
-        weights use the solutions' actual times, so steps of changing size
-        (adaptive stepping) are extrapolated correctly."""
-        t_next = ts.time + ts.sys.dt
-        t1, t2 = self.times[-1], self.times[-2]
-        a = (t_next - t1) / (t1 - t2)
-        x0 = (1.0 + a) * self.prev[-1] - a * self.prev[-2]
-        npts, deg = self.fit
-        if len(self.prev) >= npts:
-            w = lsq_weights_at(self.times[::-1][:npts], t_next, deg)
-            ix = ts.idx_x.reshape(-1)
-            xf = torch.zeros(ix.numel(), dtype=x0.dtype, device=x0.device)
-            for k, wk in enumerate(w):
-                xf.add_(self.prev[-1 - k][ix], alpha=float(wk))
-            x0[ix] = xf
-        return x0
-
-    # ---- predictor protocol ------------------------------------------------
+    # ---- predictor protocol -------------------------------------------------
     def predict(self, ts):
-        S = ts.sys
         self._pending = None
-        if len(self.prev) < 2:
-            return self.prev[-1].clone() if self.prev else None
-        x0 = self._extrapolated(ts)
-        if len(self.prev) < self.keep or not self.use_net:
-            return x0
+        if len(self._rows) < 2:
+            return self._H[self._rows[-1]].clone() if self._rows else None
+        have_cor = (self.correct or self.use_net) and self._have_corrected()
+        if not self.use_net:
+            return self._extrapolated(ts, corrected=have_cor and self.correct)
+        x_ext = self._extrapolated(ts)
+        x_cor = self._extrapolated(ts, corrected=True) if have_cor else None
+        # the net corrects the residual-corrected extrapolation where the
+        # stepper hands over corrections, else the plain one
+        base = x_cor if x_cor is not None else x_ext
+        if len(self._rows) < self.keep:
+            return base
         if not self.trained:
-            sh = self._shell_terms(ts) if S.nper else None
-            self._pending = (*self._fiber_terms(ts), sh, x0)
-            return x0
-        # trained: only the nets the gate kept are evaluated
+            self._pending = (*self._fiber_terms(ts, x_ext, x_cor), base)
+            return base
+        if self.shadow_left > 0:
+            # held-out check of the trained net on the quantity that matters:
+            # its guess's residual against the baseline's (FP32-compensated
+            # products), while the baseline guess still drives the solve
+            x_net = self._net_guess(ts, base, x_ext, x_cor)
+            S = ts.sys
+            rn = lambda x: float(torch.linalg.norm(S.rhs - S.matvec(x, precision="f32c")))
+            self.shadow.append((rn(x_net), rn(base)))
+            self.shadow_left -= 1
+            if self.shadow_left == 0:
+                net, bas = (float(np.mean(v)) for v in zip(*self.shadow))
+                self.enabled_f = net < self.gate * bas
+                self.report["fiber"].update({"held_out_guess_residual_net": net,
+                                             "held_out_guess_residual_baseline": bas,
+                                             "held_out_steps": len(self.shadow),
+                                             "enabled": self.enabled_f})
+            return base
+        if self.enabled_f:
+            return self._net_guess(ts, base, x_ext, x_cor)
+        return base
+
+    def _smoother(self, ts):
+        """Per node count N: the fibers' node indices (B, N) and the
+        least-squares projection onto the first `modes` Legendre polynomials
+        in the fiber parameter (N x N).  The fiber operator's fourth
+        derivatives amplify rough (node-to-node) errors by orders of
+        magnitude, so the net's correction is kept to smooth modes."""
+        if getattr(self, "_sm", None) is None:
+            S = ts.sys
+            self._sm = []
+            for N, bk in S.buckets.items():
+                a = bk["alpha"].cpu().numpy()
+                K = min(self.modes, N)
+                V = np.polynomial.legendre.legvander(a, K - 1)
+                Pm = V @ np.linalg.pinv(V)
+                nodes = bk["start"][:, None] + torch.arange(N, device=S.dev)[None, :]
+                self._sm.append((nodes, torch.from_numpy(Pm).to(S.dev)))
+        return self._sm
+
+    def _net_guess(self, ts, base, x_ext, x_cor):
         with torch.no_grad():
-            if self.enabled_f:
-                ff, frame = self._fiber_terms(ts)
-                c = self.net_f.forward(self.nf(ff.float())).double() * self.x_scale
-                ix = ts.idx_x
-                x0[ix] = x0[ix] + torch.einsum("nij,ni->nj", frame, c)
-            if S.nper and self.enabled_s:
-                u, inv = self._shell_terms(ts)
-                c = self.net_s.forward(self.ns(inv.float())).double() * self.c_scale
-                x0[:3 * S.nper] += (u * c[:, :, None]).sum(1).reshape(-1)
+            ff, frame = self._fiber_terms(ts, x_ext, x_cor)
+            c = self.net_f.forward(self.nf(ff.float())).double() * self.f_scale
+            d = torch.cat([torch.einsum("nij,ni->nj", frame, c[:, :3]), c[:, 3:4]], dim=1)
+            for nodes, Pm in self._smoother(ts):
+                d[nodes] = torch.einsum("mn,bnk->bmk", Pm, d[nodes])
+            x0 = base.clone()
+            ix = ts.idx_x
+            x0[ix] = x0[ix] + d[:, :3]
+            x0[ts.idx_t] = x0[ts.idx_t] + d[:, 3]
         return x0
 
-    def observe(self, ts, sol):
-        S = ts.sys
+    def observe(self, ts, sol, corr=None):
+        """Record an accepted solution and its residual correction P^-1 r."""
         pend = getattr(self, "_pending", None)
-        self.prev = (self.prev + [sol.clone()])[-self.keep:]
+        self._push(sol, corr)
         self.times = (self.times + [float(ts.time)])[-self.keep:]
         if self.trained or pend is None:
             return
-        ff, frame, sh, x_ext = pend
-        dX = (sol - x_ext)[ts.idx_x]  # (npts, 3)
-        self.samples_f.append((ff, torch.einsum("nij,nj->ni", frame, dX)))
-        if sh is not None:
-            u, inv = sh
-            dq = (sol - x_ext)[:3 * S.nper].view(S.nper, 3)
-            # least-squares coefficients of the correction in the basis u
-            G = torch.einsum("nai,nbi->nab", u, u)
-            G = G + 1e-12 * (G.diagonal(dim1=1, dim2=2).sum(-1) + 1e-300)[:, None, None] \
-                * torch.eye(3, device=u.device, dtype=u.dtype)
-            rhs = torch.einsum("nai,ni->na", u, dq)
-            c = torch.linalg.solve(G, rhs[..., None])[..., 0]
-            self.samples_s.append((inv, c))
+        ff, frame, base = pend
+        # target: the corrected solution x + P^-1 r where given (closer to
+        # the exact solution than x) minus the guess the net corrects
+        d = (sol if corr is None else sol + corr) - base
+        tgt = torch.cat([torch.einsum("nij,nj->ni", frame, d[ts.idx_x]),
+                         d[ts.idx_t][:, None]], dim=1)
+        if self.max_samples is not None and ff.shape[0] > self.max_samples:
+            pick = torch.randperm(ff.shape[0], generator=self._rng)[:self.max_samples]
+            pick = pick.to(ff.device)
+            ff, tgt = ff[pick], tgt[pick]
+        self.samples_f.append((ff, tgt))
         if len(self.samples_f) >= self.train_steps:
             self.train()
 
-    # ---- training -----------------------------------------------------------
+    # ---- training -------------------------------------------------------------
     def _split(self, samples):
         nval = max(1, int(round(self.val_frac * len(samples))))
         tr, va = samples[:-nval], samples[-nval:]
@@ -327,7 +438,7 @@ class LearnedGuess:
         Xt, Yt, Xv, Yv = self._split(samples)
         norm = Normalizer()
         norm.fit(Xt)
-        # per-output scale: the outputs may differ by orders of magnitude
+        # per-output scale: the outputs differ by orders of magnitude
         scale = Yt.abs().amax(0).clamp_min(1e-30)
         net = MLP(Xt.shape[1], n_out, seed=seed)
         net.W[2].zero_()
@@ -336,22 +447,19 @@ class LearnedGuess:
             pred = net.forward(norm(Xv)) * scale
             mse_net = float(((pred - Yv) ** 2).mean())
             mse_base = float((Yv**2).mean())
-        return net, norm, scale, mse_net, mse_base
+        return net, norm, scale.double(), mse_net, mse_base
 
     def train(self):
         torch.manual_seed(self.seed)
-        self.net_f, self.nf, self.x_scale, mn, mb = self._train_one(self.samples_f, 3, self.seed)
-        self.enabled_f = mn < self.gate * mb
-        self.report["fiber"] = {"val_mse_net(position)": mn, "val_mse_fit(position)": mb,
-                                "enabled": self.enabled_f}
-        if self.samples_s:
-            self.net_s, self.ns, self.c_scale, mn, mb = self._train_one(self.samples_s, 3,
-                                                                        self.seed + 1)
-            self.enabled_s = mn < self.gate * mb
-            self.report["shell"] = {"val_mse_net(coeff)": mn, "val_mse_extrapolation(coeff)": mb,
-                                    "enabled": self.enabled_s}
+        self.net_f, self.nf, self.f_scale, mn, mb = self._train_one(self.samples_f, 4, self.seed)
+        # first gate: the held-out steps' MSE; the second is the held-out
+        # guess residual over the next `shadow_steps` steps (predict)
+        self.enabled_f = False
+        self.shadow_left = self.shadow_steps if mn < self.gate * mb else 0
+        self.report["fiber"] = {"val_mse_net": mn, "val_mse_baseline": mb,
+                                "enabled": False, "training_steps": len(self.samples_f)}
         self.trained = True
-        self.samples_f, self.samples_s = [], []
+        self.samples_f = []
 
     def _fit(self, net, X, Y):
         n = X.shape[0]

@@ -77,8 +77,11 @@ class Shell:
     applies lu_solve, the same linear map.  The inverse is applied in FP64 by
     the libf3d dense GEMV: bandwidth-bound instead of a latency-bound pair of
     triangular sweeps, and free of BLAS/rocSOLVER calls, whose workspaces a
-    HIP graph cannot keep.  pc_f32 is accepted and ignored.  pc='block'
-    inverts only each node's 3x3 diagonal block, -E_i/w_i - I/w_i + n_i n_i^T
+    HIP graph cannot keep.  pc_f32=True (the default) keeps an FP32 copy of
+    the inverse as well, applied with FP64 arithmetic when the caller asks
+    for the FP32 preconditioner (the fast mode, beside its FP32 fiber
+    factors): half the HBM traffic per apply.  pc='block' inverts only each
+    node's 3x3 diagonal block, -E_i/w_i - I/w_i + n_i n_i^T
     (0.1 ms per apply at 32768 nodes; about twice the GMRES iterations of the
     dense form).  The dense inverse grows as (3N)^2: 4.6 GB and 7 s of set-up
     at 8000 nodes, 77 GB and minutes of set-up at 32768, so pc='auto' picks
@@ -87,7 +90,7 @@ class Shell:
     DENSE_MAX = 8192
 
     def __init__(self, x, normals, weights, eta=1.0, reg=5e-3, eps=1e-5, pc="auto",
-                 pc_f32=False):
+                 pc_f32=True):
         dev = device()
         self.n = x.shape[0]
         self.pos = torch.from_numpy(np.ascontiguousarray(x, np.float64)).to(dev)
@@ -104,11 +107,13 @@ class Shell:
         if pc == "auto":
             pc = "dense" if self.n <= self.DENSE_MAX else "block"
         self.pc = pc
-        self.minv = None
+        self.minv = self.minv32 = None
         if pc == "dense":
             # FP64 inverse applied with the libf3d dense GEMV (no BLAS call,
             # so the preconditioner can be recorded in a HIP graph)
             self.minv = DenseGemv(dense_inverse(self.dense_matrix()))
+            if pc_f32:
+                self.minv32 = DenseGemv(self.minv.M.float())
         elif pc == "block":
             E = torch.stack(self.e, dim=2)  # E[i, a, c] = e_c[i, a]
             winv = (1.0 / self.w)[:, None, None]
@@ -143,8 +148,8 @@ class Shell:
                                precision=precision, out=out)
 
     def precond(self, r, out, f32=False):
-        """Shell block of P^-1 (f32 is accepted for symmetry with the fiber
-        factors; the shell inverse is applied in FP64, an HBM-bound GEMV)."""
+        """Shell block of P^-1: the dense inverse (an HBM-bound GEMV), from its
+        FP32 copy when f32 is set and the copy exists, else in FP64."""
         if self.pc == "block":
             # per node: out_i = D_i^-1 r_i, written as elementwise products
             D, rv = self.Dinv, r.view(self.n, 3)
@@ -152,7 +157,7 @@ class Shell:
                                       + D[:, :, 2] * rv[:, 2:3])
         else:
             src = r if (r.is_contiguous() and r.data_ptr() != out.data_ptr()) else r.clone()
-            self.minv(src, out)
+            (self.minv32 if f32 and self.minv32 is not None else self.minv)(src, out)
         return out
 
 
@@ -442,6 +447,7 @@ class FiberSystem:
                 "D0": torch.stack(bk["D0"]).contiguous(),
                 "PX": P[:N - 4, :N].contiguous(), "PT": P[3 * (N - 4):, 3 * N:].contiguous(),
                 "length": self.lengths[bk["ids_t"]].contiguous(),
+                "rhs_off": ti(self.fib0 + 4 * self.offs[ids]),
             }
         self._setup_attached(attached)
         self._setup_cortex(cortex)
@@ -675,9 +681,10 @@ class FiberSystem:
             self.xss = torch.empty((self.npts, 3), dtype=torch.float64, device=dev)
         A_buf, F_buf, Q_buf = self.A_buf, self.F_buf, self.Q_buf
         self._rhs_parts = {}
-        self._ctx = {}
+        self._ctx_data = {}
         if self.assembly == "hip":
             self._assemble_hip(A_buf, F_buf, Q_buf)
+            self._ctx_data = None  # per-bucket views, formed on first use
             buckets = {}
         else:
             buckets = self.buckets
@@ -744,7 +751,7 @@ class FiberSystem:
             w = bk["w0"][None, :] * (self.lengths[ids_t][:, None] / 2.0)
             self.weights[node.reshape(-1)] = w.reshape(-1)
             self.xs[node.reshape(-1)] = xs.reshape(-1, 3)
-            self._ctx[N] = (node, X, xs, xss, D1, P, B)
+            self._ctx_data[N] = (node, X, xs, xss, D1, P, B)
         self._build_rhs()
         if self.att or self.cor:
             self._apply_clamped()
@@ -779,10 +786,24 @@ class FiberSystem:
                 float(self.penalty), L.ptr(A_buf), L.ptr(F_buf), L.ptr(Q_buf[0]), L.ptr(Q_buf[1]),
                 L.ptr(Q_buf[2]), L.ptr(self.xs), L.ptr(self.xss), L.ptr(self.weights), _stream())
             L.check(rc, "f3d_fiber_assemble")
-            node = bk["start"][:, None] + torch.arange(N, device=self.dev)[None, :]
-            sc = (2.0 / self.lengths[bk["ids_t"]])[:, None, None]
-            self._ctx[N] = (node, self.X[node], self.xs[node], self.xss[node],
-                            bk["D0"][0][None] * sc, bk["P"], B)
+
+    @property
+    def _ctx(self):
+        """Per node count N: (node indices (B, N), X, xs, xss at them (B, N, 3),
+        scaled D_1 (B, N, N), projection P, fiber count B) of the current
+        assembly; the HIP path forms them only when something asks."""
+        if self._ctx_data is None:
+            ctx = {}
+            for N, bk in self.buckets.items():
+                if "node" not in bk:
+                    bk["node"] = bk["start"][:, None] + torch.arange(N, device=self.dev)[None, :]
+                    sc = (2.0 / self.lengths[bk["ids_t"]])[:, None, None]
+                    bk["D1"] = bk["D0"][0][None] * sc
+                node = bk["node"]
+                ctx[N] = (node, self.X[node], self.xs[node], self.xss[node], bk["D1"], bk["P"],
+                          len(bk["ids"]))
+            self._ctx_data = ctx
+        return self._ctx_data
 
     def _build_rhs(self):
         """RHS: explicit flows of the plus-end forces and per-fiber RHS
@@ -816,6 +837,19 @@ class FiberSystem:
                 B.explicit(self.shell.pos, out=vfs, accumulate=True)
             rhs[:3 * self.nper] = -vfs.reshape(-1)
         dt, c0, c1, p = self.dt, self.c0, self.c1, self.penalty
+        if self.assembly == "hip":
+            # fused per-fiber RHS rows, one launch per node count
+            lib = L.load()
+            for N, bk in self.buckets.items():
+                t = bk["asm"]
+                rc = lib.f3d_fiber_rhs(
+                    len(bk["ids"]), N, L.ptr(t["node0"]), L.ptr(t["length"]), L.ptr(t["rhs_off"]),
+                    L.ptr(self.X), L.ptr(self.xs), L.ptr(self.xss), L.ptr(vff), L.ptr(t["D0"]),
+                    L.ptr(t["PX"]), L.ptr(t["PT"]), float(dt), float(c0), float(c1), float(p),
+                    float(self.end_force), L.ptr(rhs), _stream())
+                L.check(rc, "f3d_fiber_rhs")
+            self.rhs = rhs
+            return
         for N, (node, X, xs, xss, D1, P, B) in self._ctx.items():
             bk = self.buckets[N]
             fk = force[node]  # (B, N, 3)

@@ -64,6 +64,7 @@ class StepStats:
     accepted: bool = True
     dt: float = 0.0                  # the step size this solve used
     inextensibility: float = 0.0     # max | |x_s| - 1 | of the new configuration
+    guess_residual: float = None     # ||b - A x0|| / ||b|| of an accepted initial guess
 
 
 def periphery_collision(X, a, b, c):
@@ -220,10 +221,12 @@ class TimeStepper:
         worst = max(v for _, v in self._inner_meas)
         self.inner_rel_err = max(self.inner_rel_err_floor, self.inner_rel_err_safety * worst)
 
-    def inextensibility_error(self):
+    def inextensibility_error(self, as_tensor=False):
         """max over all fiber nodes of | |x_s| - 1 | for the assembled
-        configuration (tstep.check_error, tstep.py:1874-1891)."""
-        return float((torch.linalg.norm(self.sys.xs, dim=1) - 1.0).abs().max())
+        configuration (tstep.check_error, tstep.py:1874-1891); as a device
+        scalar when as_tensor is set (no host synchronisation)."""
+        e = (torch.linalg.norm(self.sys.xs, dim=1) - 1.0).abs().max()
+        return e if as_tensor else float(e)
 
     def collision(self):
         """A fiber node on or outside the periphery (the fiber-cortex part of
@@ -266,6 +269,12 @@ class TimeStepper:
             self._note_inner(res.inner_err)
         self._last_its = res.iterations
         sol = res.x
+        # the residual correction P^-1 r for the predictor, with this step's
+        # preconditioner (before the re-assembly below)
+        corr = None
+        if (self.predictor is not None and getattr(self.predictor, "wants_correction", False)
+                and res.resid is not None):
+            corr = S.precond(res.resid)
         body_old = S.bodies.state() if S.bodies is not None else None
         if self.record:
             # S.rhs and sol are fresh tensors of this step; X is updated in place
@@ -283,7 +292,9 @@ class TimeStepper:
         refactor = self.nsteps % self.lu_every == 0
         S.prepare(refactor=refactor)
         self._changed = refactor
-        err = self.inextensibility_error()
+        # read at once only when the step size depends on it, else after
+        # the step's closing synchronisation
+        err = self.inextensibility_error(as_tensor=not self.adaptive)
         accept, dt_new = True, dt
         if self.adaptive:
             accept, dt_new = adaptive_decision(err, self.collision(), dt, self.tol_tstep,
@@ -291,7 +302,10 @@ class TimeStepper:
         if accept:
             self.time += dt
             if self.predictor is not None:
-                self.predictor.observe(self, sol)
+                if getattr(self.predictor, "wants_correction", False):
+                    self.predictor.observe(self, sol, corr)
+                else:
+                    self.predictor.observe(self, sol)
             self.prev_solution = sol
             self.shell_density = sol[:3 * S.nper].clone()
             self.tension = sol[self.idx_t]
@@ -308,8 +322,10 @@ class TimeStepper:
             S.prepare(refactor=True)
             self._changed = True
         torch.cuda.synchronize()
+        err = float(err)
         st = StepStats(res.iterations, res.restarts, res.residual, time.perf_counter() - t0,
-                       res.guess_used, res.residual_kind, res.residual_bound, accept, dt, err)
+                       res.guess_used, res.residual_kind, res.residual_bound, accept, dt, err,
+                       res.history[0] if res.guess_used and res.history else None)
         self.history.append(st)
         return st
 

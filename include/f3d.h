@@ -262,6 +262,24 @@ int f3d_fiber_assemble(int64_t nfib, int32_t N, const int64_t* node0, const doub
                        double* Q2, double* xs_out, double* xss_out, double* w_out, void* stream);
 
 /*
+ * Per-fiber right-hand side rows for nfib fibers of one node count N
+ * (5 <= N <= 64), FP64, one workgroup per fiber, after f3d_fiber_assemble
+ * (xs, xss at the nodes) and the explicit flow v (npts x 3, AoS) of the
+ * plus-end forces F_end x_s(L).  Writes the fiber's 4N rows at
+ * rhs + rhs_off[f]: the projected collocation rows P R and the 14 end rows
+ * (the plus-end force and its tangential part).  D0 as in
+ * f3d_fiber_assemble (only D_1 is read).  Replaces fiber.compute_RHS with the
+ * plus-end force (python/fiber/fiber.py:473-582), the RHS part of
+ * apply_BC_rectangular (:810-961) and the per-fiber RHS assembly of
+ * python/tstep/tstep.py:794-846 and tstep_utils.py:1238-1336.
+ */
+int f3d_fiber_rhs(int64_t nfib, int32_t N, const int64_t* node0, const double* length,
+                  const int64_t* rhs_off, const double* X, const double* xs, const double* xss,
+                  const double* v, const double* D0, const double* PX, const double* PT,
+                  double dt, double c0, double c1, double penalty, double end_force,
+                  double* rhs, void* stream);
+
+/*
  * y_b = alpha * M_b x_b + beta * y_b for a batch of dense COLUMN-major blocks
  * of shape m[b] x k[b] (leading dimension m[b]) stored at M + m_off[b]; x at
  * x_off[b], y at y_off[b] (y must not alias x).  m[b] <= mmax <= 1024.
@@ -285,6 +303,15 @@ int f3d_block_gemv_batched(int64_t nb, int32_t mmax, const int32_t* m, const int
  */
 int f3d_dense_gemv_batched(int64_t count, int32_t m, const double* M, const double* x,
                            double* y, void* stream);
+
+/*
+ * The same with FP32 matrix entries (FP64 x, products, sums and y): half the
+ * HBM traffic of the FP64 sweep.  Applies the shell inverse as the fast
+ * mode's preconditioner, beside the FP32 fiber LU factors; the reference's
+ * shell lu_solve (python/tstep/tstep.py:957-958) is the map it approximates.
+ */
+int f3d_dense_gemv_batched_f32(int64_t count, int32_t m, const float* M, const double* x,
+                               double* y, void* stream);
 
 /* ---- restarted GMRES on the device ----------------------------------- */
 /*

@@ -39,7 +39,7 @@ class HydroSystem:
     def __init__(self, fibers, shell=None, eta=1.0, dt=1e-3, E=0.01, epsilon=1e-3,
                  penalty=500.0, nfd=4, end_force=1.0, discretisation="reference",
                  lengths=None, bodies=None, body_force=None, attached=None, hinged=None,
-                 cortex=False):
+                 cortex=False, shell_cache=None):
         """fibers: list of (N_k,3) arrays; shell: (x, normals, weights) or None;
         bodies: list of (centre (3,), nodes (Nb,3), outward normals, weights);
         body_force: optional (nbody, 6) external force and torque per body;
@@ -49,7 +49,9 @@ class HydroSystem:
         end is held in place with free rotation, as a fiber that reached the
         periphery (tstep.py:886-888); cortex: every fiber not attached to a
         body has its minus end clamped in place (attached_to_cortex,
-        tstep.py:881-883)."""
+        tstep.py:881-883).  shell_cache: the `shell_state()` of an earlier
+        system with the same shell, whose one-time set-up (subtraction
+        vectors, dense shell matrix and its LU, tstep.py:278-356) is reused."""
         self.eta, self.dt, self.E, self.penalty = eta, dt, E, penalty
         self.c0, self.c1 = OF.constants(epsilon, eta)
         self.fibers = [np.asarray(x, float) for x in fibers]
@@ -75,7 +77,12 @@ class HydroSystem:
         self.cortex = sorted(set(range(len(self.fibers))) - self.clamped) if cortex else []
         self.clamped |= set(self.cortex)
         self.hinged = set() if hinged is None else {int(k) for k in hinged}
+        self._shell_cache = shell_cache
         self._setup(end_force)
+
+    def shell_state(self):
+        """The shell's one-time set-up, for shell_cache."""
+        return {"e": (self.ex, self.ey, self.ez), "M": self.M_shell, "lu": self.lu_shell}
 
     # -------------------------------------------------------------------
     def _setup(self, end_force):
@@ -131,7 +138,10 @@ class HydroSystem:
             vfib2shell = K.oseen_kernel_source_target(self.trg_fib, xs_, fw, eta) \
                 if self.npts else np.zeros(3 * self.nper)
             rhs[:3 * self.nper] = -(vfib2shell + body_explicit(xs_))
-            self.ex, self.ey, self.ez = P.singularity_subtraction_vectors(xs_, ns_, ws_, eta)
+            if self._shell_cache is not None:
+                self.ex, self.ey, self.ez = self._shell_cache["e"]
+            else:
+                self.ex, self.ey, self.ez = P.singularity_subtraction_vectors(xs_, ns_, ws_, eta)
         # per-fiber square systems (get_fibers_and_bodies_matrices)
         self.A_sq = []
         self.Q = []
@@ -166,8 +176,11 @@ class HydroSystem:
         self.lu_fib = [sla.lu_factor(A, check_finite=False) for A in self.A_sq]
         if self.shell is not None:
             xs_, ns_, ws_ = self.shell
-            self.M_shell = P.shell_matrix(xs_, ns_, ws_, self.ex, self.ey, self.ez, eta)
-            self.lu_shell = sla.lu_factor(self.M_shell)
+            if self._shell_cache is not None:
+                self.M_shell, self.lu_shell = self._shell_cache["M"], self._shell_cache["lu"]
+            else:
+                self.M_shell = P.shell_matrix(xs_, ns_, ws_, self.ex, self.ey, self.ez, eta)
+                self.lu_shell = sla.lu_factor(self.M_shell)
         # body preconditioner blocks [M - I_sub, -K; -K^T, I] (tstep_utils.py:1341-1387)
         self.K = []
         self.lu_body = []

@@ -177,3 +177,24 @@ def test_dense_gemv_batched(count, m, offset):
     torch.cuda.synchronize()
     ref = np.einsum("bij,bj->bi", M, x.reshape(count, m)).ravel()
     assert rel2(y.cpu().numpy(), ref) < 1e-13
+
+
+@pytest.mark.parametrize("count,m,offset", [(1, 3072, 0), (3, 1806, 0), (2, 7, 0), (2, 132, 1),
+                                            (1, 1, 0)])
+def test_dense_gemv_batched_f32_entries(count, m, offset):
+    """FP32 matrix entries, FP64 vector and arithmetic: equal to the FP64
+    product of the FP32-rounded matrix to FP64 rounding, on the four-floats-
+    per-lane path (m a multiple of 4, aligned), odd sizes and an x that is not
+    16-byte aligned (one entry per lane)."""
+    print(f"case dense gemv f32 count={count} m={m} offset={offset}", flush=True)
+    rng = np.random.default_rng(m + count + 1)
+    M32 = rng.normal(size=(count, m, m)).astype(np.float32)
+    x = rng.normal(size=count * m)
+    g = DenseGemv(torch.from_numpy(M32).cuda())
+    xbuf = torch.zeros(count * m + offset, dtype=torch.float64, device="cuda")
+    xbuf[offset:] = torch.from_numpy(x)
+    y = torch.full((count * m,), float("nan"), dtype=torch.float64, device="cuda")
+    g(xbuf[offset:], y)
+    torch.cuda.synchronize()
+    ref = np.einsum("bij,bj->bi", M32.astype(np.float64), x.reshape(count, m)).ravel()
+    assert rel2(y.cpu().numpy(), ref) < 1e-13

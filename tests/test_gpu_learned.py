@@ -37,29 +37,47 @@ def test_bad_guess_falls_back_to_zero_start():
     assert r3.guess_used and r3.iterations == 0
 
 
-def test_learned_guess_reduces_iterations():
+def test_learned_guess_beats_extrapolation_on_held_out_steps():
+    """A time series: the learned predictor trains its fiber net on the first
+    `train` steps and checks it on 4 gate steps; then all predictors run the
+    same number of further (held-out) steps.  The learned predictor's
+    baseline is the extrapolation of the residual-corrected history
+    (x + P^-1 r); on the held-out steps its guesses have at least 10% smaller
+    residuals and need at least 10% fewer GMRES iterations than the plain
+    extrapolation, and never do worse than the corrected extrapolation alone:
+    the net is kept only if it beats its baseline on both gates (validation
+    MSE, then guess residuals, below 0.9 times the baseline's)."""
     print("case learned time series", flush=True)
-    fibers, shell = _setup()
-    train = 24
-    runs, reports = {}, {}
-    for name, pred in (("zero", None),
-                       ("extrapolated", LearnedGuess(train_steps=10**9, use_net=False)),
-                       ("learned", LearnedGuess(train_steps=train, epochs=60))):
+    fibers, shell = _setup(nfib=160, nper=1600, seed=3)
+    train, held = 30, 40
+    runs, r0s, reports = {}, {}, {}
+    preds = (("zero", None),
+             ("extrapolated", LearnedGuess(train_steps=10**9, use_net=False)),
+             ("corrected", LearnedGuess(train_steps=10**9, use_net=False, correct=True)),
+             ("learned", LearnedGuess(train_steps=train, epochs=60)))
+    for name, pred in preds:
         ts = TimeStepper(fibers, shell, mode="fast", tol=1e-8, predictor=pred)
-        its = []
-        for _ in range(train + 16):
-            st = ts.step()
-            assert st.residual <= 1e-8
-            its.append(st.iterations)
-        runs[name] = float(np.mean(its[-8:]))
+        warm = 8 + train + 4  # the learned predictor's history, training and gate steps
+        stats = [ts.step() for _ in range(warm + held)]
+        assert all(st.residual <= 1e-8 for st in stats)
+        tail = stats[warm:]
+        runs[name] = float(np.mean([st.iterations for st in tail]))
+        r0s[name] = float(np.mean([st.guess_residual for st in tail])) if pred else None
         if pred is not None and pred.use_net:
             assert pred.trained
             reports[name] = pred.report
-    print("mean GMRES iterations over the last 8 steps:", runs, reports)
+    print("held-out mean GMRES iterations:", runs, "guess residuals:", r0s, reports, flush=True)
     assert runs["extrapolated"] < runs["zero"]
-    # the gate keeps a net only if it beats the extrapolation on held-out steps
-    for part in reports["learned"].values():
-        if part["enabled"]:
-            vals = list(part.values())
-            assert vals[0] < 0.9 * vals[1]
-    assert runs["learned"] <= runs["extrapolated"] + 0.25
+    assert r0s["learned"] <= 0.9 * r0s["extrapolated"]
+    assert runs["learned"] <= 0.9 * runs["extrapolated"]
+    # the gated net never makes the corrected extrapolation worse (the bar
+    # leaves room for the trajectories' atomic-summation differences)
+    assert r0s["learned"] <= 1.05 * r0s["corrected"]
+    rep = reports["learned"]["fiber"]
+    if rep["enabled"]:
+        assert rep["val_mse_net"] < 0.9 * rep["val_mse_baseline"]
+        assert rep["held_out_guess_residual_net"] < 0.9 * rep["held_out_guess_residual_baseline"]
+    elif "held_out_guess_residual_net" in rep:
+        assert rep["held_out_guess_residual_net"] >= 0.9 * rep["held_out_guess_residual_baseline"]
+    else:
+        assert rep["val_mse_net"] >= 0.9 * rep["val_mse_baseline"]

@@ -259,3 +259,15 @@ def test_periphery_collision_is_the_ellipsoid_inequality():
         assert not OT.periphery_collision(inside, a, b, c)
         mixed = np.vstack([inside, surf[:1] * 1.01])
         assert OT.periphery_collision(mixed, a, b, c)
+
+
+def test_shell_cache_reproduces_the_system(small_system):
+    """A system built on an earlier one's shell set-up (shell_cache) equals
+    one built from scratch."""
+    S = small_system
+    T = OT.HydroSystem(S.fibers, S.shell, eta=1.0, dt=1e-3, E=0.01,
+                       shell_cache=S.shell_state())
+    x = np.random.default_rng(5).normal(size=S.size)
+    assert np.array_equal(T.rhs, S.rhs)
+    assert np.array_equal(T.matvec(x), S.matvec(x))
+    assert np.array_equal(T.precond(x), S.precond(x))

@@ -1,6 +1,7 @@
 """Host checks of the initial-guess extrapolation weights."""
 import numpy as np
 import pytest
+import torch
 
 from fiber3d_amd.learned import lsq_weights
 
@@ -73,3 +74,60 @@ def test_fibers_reaching_periphery():
         assert fibers_reaching_periphery(fibers, a, b, c) == [1, 2, 3]
         # without the margin only a node on or outside the periphery counts
         assert fibers_reaching_periphery(fibers, a, b, c, margin=0.0) == [3]
+
+
+class _Sys:
+    def __init__(self, dt):
+        self.dt = dt
+
+
+class _Stepper:
+    """The parts of TimeStepper the extrapolating predictor reads."""
+
+    def __init__(self, n, fiber_x, dt):
+        self.time = 0.0
+        self.sys = _Sys(dt)
+        self.idx_x = torch.as_tensor(fiber_x, dtype=torch.long).reshape(-1, 3)
+        self.n = n
+
+
+@pytest.mark.parametrize("correct", [False, True])
+def test_ring_extrapolation_matches_direct_formula(correct):
+    """The predictor's history ring and its one-product extrapolation give,
+    on the host, the direct formula: 2 p1 - p2 (at equal steps) for every
+    component, and the least-squares quadratic through the last six values
+    for the fiber positions; exact for quadratic histories at uneven steps;
+    with corrections, the same applied to x + c.  More solutions than the
+    ring holds wrap around without changing the result."""
+    from fiber3d_amd.learned import LearnedGuess, lsq_weights_at
+    n = 40
+    fiber_x = np.arange(6, 30)  # 8 nodes x 3 coordinates
+    rng = np.random.default_rng(5)
+    c0, c1, c2 = rng.normal(size=(3, n))
+    cc = rng.normal(size=n) * 1e-3
+    x = lambda t: c0 + c1 * t + c2 * t * t
+    steps = [0.1, 0.1, 0.07, 0.12, 0.1, 0.09, 0.11, 0.1, 0.05]
+    pred = LearnedGuess(train_steps=10**9, use_net=False, correct=correct)
+    ts = _Stepper(n, fiber_x, steps[0])
+    times = []
+    for k, h in enumerate(steps):
+        ts.time += h
+        times.append(ts.time)
+        sol = torch.from_numpy(x(ts.time))
+        corr = torch.from_numpy(cc * (k + 1)) if correct else None
+        pred.observe(ts, sol, corr)
+        ts.sys.dt = steps[(k + 1) % len(steps)]
+        if k == 0:
+            continue
+        got = pred.predict(ts).numpy()
+        hist = [x(t) + (cc * (j + 1) if correct else 0.0) for j, t in enumerate(times)]
+        t1, t2, tn = times[-1], times[-2], ts.time + ts.sys.dt
+        a = (tn - t1) / (t1 - t2)
+        want = (1 + a) * hist[-1] - a * hist[-2]
+        if len(times) >= 6:
+            w = lsq_weights_at(np.array(times[::-1][:6]), tn, 2)
+            want[fiber_x] = sum(w[j] * hist[-1 - j][fiber_x] for j in range(6))
+            if not correct:
+                np.testing.assert_allclose(got[fiber_x], x(tn)[fiber_x], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+    assert pred.wants_correction == correct
