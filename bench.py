@@ -222,16 +222,18 @@ def system_leg(cfg_id, steps, train_steps=0, scale=None, shell_pc="auto", with_b
                       LearnedGuess(train_steps=10**9, use_net=False, correct=True)))
         modes.append(("learned_fast", "fast", LearnedGuess(train_steps=train_steps, epochs=20,
                                                           max_samples=32768)))
+    learned_warm = max([p.warm_steps for _, _, p in modes if p is not None] or [1])
     for name, mode, pred in modes:
         ts = TimeStepper(fibers, shell, mode=mode, eta=cfg.eta, dt=cfg.dt, E=spec.E,
                          epsilon=spec.epsilon, tol=cfg.gmres_tol, restart=cfg.restart,
                          maxiter=cfg.maxiter, predictor=pred, lengths=np.full(len(fibers), spec.length),
                          bodies=bodies, graph=graph, record=mode == "fast")
         # a predictor needs `keep` solutions before its full guess; the learned
-        # mode then collects train_steps samples, trains and checks the net on
-        # 4 further steps, all before timing starts.  Every guessing mode times the same steps: those after the
-        # learned mode's history and training window (held-out steps).
-        warm = 1 if pred is None else pred.keep + 2 + train_steps + 4
+        # mode then picks its baseline (4 steps), collects train_steps
+        # samples, trains and checks the net on 4 further steps, all before
+        # timing starts.  Every guessing mode times the same steps: those
+        # after the learned mode's window (held-out steps).
+        warm = 1 if pred is None else learned_warm
         for _ in range(warm):
             ts.step()
         ts.records = []

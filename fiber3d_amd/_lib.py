@@ -33,6 +33,11 @@ SIGNATURES = {
     "f3d_last_error": (ctypes.c_char_p, []),
     "f3d_abi_version": (ctypes.c_int, []),
     "f3d_pairsum_set_launch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i32, _i32]),
+    "f3d_pairsum_set_launch_bucket": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i32, _i32,
+                                                     _i32, _i32]),
+    "f3d_pairsum_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _i64, _c_p]),
+    "f3d_lu_set_launch": (ctypes.c_int, [ctypes.c_int, _i32, _i32]),
+    "f3d_gemm_set_stream": (ctypes.c_int, [_i32, _i32]),
     "f3d_pairsum_timer": (ctypes.c_int, [ctypes.c_int]),
     "f3d_pairsum_timer_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_int32)]),

@@ -733,6 +733,13 @@ lu_solve_kernel(const int* __restrict__ n_arr, const long* __restrict__ a_off,
 // lu_solve_kernel with refine = 0 (FP64 substitutions in the same order).
 constexpr int kLdsSolveBytes = 64 * 1024;
 
+// Launch choices set by the autotuner through f3d_lu_set_launch, per
+// precision (0 = FP64, 1 = FP32): waves per CU of the one-wave blocked
+// factorisation, and whether solves without refinement stage the factor in
+// LDS when it fits.
+static int g_lu_per_cu[2] = {4, 8};
+static int g_lu_solve_staged[2] = {1, 1};
+
 // Substitutions out of LDS.  The loops run over the 64-row slot ms of the
 // pivot row k as an unrolled constant, so every register of w is addressed
 // statically, and the columns of the next kLdsGroup pivots are read while the
@@ -1054,7 +1061,7 @@ static int num_cus() {
 int launch_lu_wave(int precision, int64_t nb, const int32_t* n, const int64_t* a_off,
                    const int64_t* x_off, const int32_t* order, const double* A, double* lu64,
                    float* lu32, double* rs, double* cs, int32_t* piv, int32_t* perm,
-                   int32_t* info, int num_cu, hipStream_t st);
+                   int32_t* info, int num_cu, int per_cu, hipStream_t st);
 
 }  // namespace f3d
 
@@ -1085,7 +1092,7 @@ extern "C" int f3d_lu_factor_batched(int precision, int64_t nb, int32_t nmax, co
       F3D_REQUIRE(lu32 && rs && cs, F3D_EINVAL, "lu_factor: lu32/rs/cs is NULL");
     }
     return launch_lu_wave(precision, nb, n, a_off, x_off, order, A, lu64, lu32, rs, cs, piv,
-                          perm, info, num_cus(), st);
+                          perm, info, num_cus(), g_lu_per_cu[precision == F3D_F64 ? 0 : 1], st);
   }
   const unsigned grid = (unsigned)(nb < num_cus() ? nb : num_cus());
   if (precision == F3D_F64) {
@@ -1134,6 +1141,7 @@ extern "C" int f3d_lu_solve_batched(int precision, int64_t nb, int32_t nmax, con
   // F3D_LU_SOLVE_STREAM=1 keeps the streaming kernel (for comparison)
   const char* force_stream = getenv("F3D_LU_SOLVE_STREAM");
   const bool staged = refine == 0 && lds <= (size_t)kLdsSolveBytes &&
+                      g_lu_solve_staged[precision == F3D_F64 ? 0 : 1] &&
                       !(force_stream && *force_stream && *force_stream != '0');
   if (staged && precision == F3D_F64) {
     F3D_REQUIRE(lu64 != nullptr, F3D_EINVAL, "lu_solve: lu64 is NULL");
@@ -1158,6 +1166,21 @@ extern "C" int f3d_lu_solve_batched(int precision, int64_t nb, int32_t nmax, con
                (const long*)a_off, (const long*)x_off, A, lu32, rs, cs, perm, x, y, refine, tol,
                iters);
   }
+  return F3D_OK;
+}
+
+extern "C" int f3d_lu_set_launch(int precision, int32_t factor_waves_per_cu,
+                                 int32_t solve_staged) {
+  F3D_REQUIRE(precision == F3D_F64 || precision == F3D_F32C, F3D_EINVAL,
+              "lu_set_launch: unknown precision %d", precision);
+  const int pp = precision == F3D_F64 ? 0 : 1;
+  const int cap = pp == 0 ? 4 : 8;
+  F3D_REQUIRE(factor_waves_per_cu >= 0 && factor_waves_per_cu <= cap, F3D_EINVAL,
+              "lu_set_launch: waves per CU %d outside [0, %d]", factor_waves_per_cu, cap);
+  F3D_REQUIRE(solve_staged == 0 || solve_staged == 1, F3D_EINVAL,
+              "lu_set_launch: solve_staged must be 0 or 1");
+  g_lu_per_cu[pp] = factor_waves_per_cu ? factor_waves_per_cu : cap;
+  g_lu_solve_staged[pp] = solve_staged;
   return F3D_OK;
 }
 

@@ -510,9 +510,9 @@ lu_wave_f32_kernel(int nb, const int* __restrict__ n_arr, const long* __restrict
 int launch_lu_wave(int precision, int64_t nb, const int32_t* n, const int64_t* a_off,
                    const int64_t* x_off, const int32_t* order, const double* A, double* lu64,
                    float* lu32, double* rs, double* cs, int32_t* piv, int32_t* perm,
-                   int32_t* info, int num_cu, hipStream_t st) {
-  // waves in flight per CU: LDS holds 4 FP64 panels (35 KB each) or 9 FP32
-  const int per_cu = precision == F3D_F64 ? 4 : 8;
+                   int32_t* info, int num_cu, int per_cu, hipStream_t st) {
+  // per_cu: waves in flight per CU (LDS holds at most 4 FP64 panels of 35 KB
+  // or 9 FP32 ones)
   const long g = (long)per_cu * num_cu;
   const unsigned grid = (unsigned)(nb < g ? nb : g);
   if (precision == F3D_F64) {

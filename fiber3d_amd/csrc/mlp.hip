@@ -233,6 +233,17 @@ gemm_bf16_stream_kernel(int M, int N, int K, const float* __restrict__ A, long s
 
 using namespace f3d;
 
+// Whether the streaming kernel serves N = 128 / N = 256 (set by the
+// autotuner through f3d_gemm_set_stream).
+static int g_gemm_stream[2] = {1, 1};
+
+extern "C" int f3d_gemm_set_stream(int32_t n, int32_t enable) {
+  F3D_REQUIRE(n == 128 || n == 256, F3D_EINVAL, "gemm_set_stream: N must be 128 or 256");
+  F3D_REQUIRE(enable == 0 || enable == 1, F3D_EINVAL, "gemm_set_stream: enable must be 0 or 1");
+  g_gemm_stream[n == 256 ? 1 : 0] = enable;
+  return F3D_OK;
+}
+
 extern "C" int f3d_gemm_bf16(int epi, int64_t M, int64_t N, int64_t K, int32_t ksplit,
                              const float* A, int64_t sam, int64_t sak, const float* B,
                              int64_t sbk, int64_t sbn, float* C, int64_t scm, int64_t scn,
@@ -253,7 +264,8 @@ extern "C" int f3d_gemm_bf16(int epi, int64_t M, int64_t N, int64_t K, int32_t k
   // the streaming path: row-major A (unit K stride) and C (unit N stride),
   // N in {128, 256}, K <= 256, no K split
   const bool stream_ok = epi != EPI_ATOMIC && ksplit == 1 && sak == 1 && scn == 1 &&
-                         (N == 128 || N == 256) && K <= 256 && M >= kSM;
+                         ((N == 128 && g_gemm_stream[0]) || (N == 256 && g_gemm_stream[1])) &&
+                         K <= 256 && M >= kSM;
   if (stream_ok) {
     const dim3 g((unsigned)ceil_div(M, kSM));
 #define F3D_S(E, T)                                                                            F3D_LAUNCH((gemm_bf16_stream_kernel<E, T>), g, dim3(256), 0, st, (int)M, (int)N, (int)K, A,              (long)sam, B, (long)sbk, (long)sbn, C, (long)scm, bias, Z, Zin, alpha)

@@ -515,6 +515,17 @@ static int env_int(const char* name, int dflt) {
 // f3d_pairsum_set_launch; F3D_PAIRSUM_K32 / _K64 / _WAVES override.
 static int g_launch_k[3][2] = {{2, 2}, {2, 2}, {2, 2}};
 static int g_launch_waves[3][2] = {{8192, 8192}, {8192, 8192}, {8192, 8192}};
+// Per size bucket: (ceil(log2 nt), ceil(log2 ns)), each clamped to
+// [kLgMin, kLgMax]; k == 0 leaves the bucket on the (kernel, precision) shape.
+constexpr int kLgMin = 8, kLgMax = 22, kLgN = kLgMax - kLgMin + 1;
+static int g_bucket_k[3][2][kLgN][kLgN];
+static int g_bucket_waves[3][2][kLgN][kLgN];
+
+static int lg_bucket(int64_t n) {
+  int l = 0;
+  while (l < 62 && (1ll << l) < n) ++l;
+  return (l < kLgMin ? kLgMin : (l > kLgMax ? kLgMax : l)) - kLgMin;
+}
 
 // Event pairs around the pair kernel, recorded while f3d_pairsum_timer is on.
 constexpr int kTimerSlots = 4096;
@@ -530,11 +541,15 @@ static Plan make_plan(int kernel, int precision, int64_t ns, int64_t nt) {
   Plan pl;
   const int kk = (kernel >= 0 && kernel < 3) ? kernel : 0;
   const int pp = precision == F3D_F64 ? 0 : 1;
-  pl.K = env_int(precision == F3D_F64 ? "F3D_PAIRSUM_K64" : "F3D_PAIRSUM_K32", g_launch_k[kk][pp]);
+  const int bt = lg_bucket(nt), bs = lg_bucket(ns);
+  const bool tuned = g_bucket_k[kk][pp][bt][bs] != 0;
+  const int k0 = tuned ? g_bucket_k[kk][pp][bt][bs] : g_launch_k[kk][pp];
+  const int w0 = tuned ? g_bucket_waves[kk][pp][bt][bs] : g_launch_waves[kk][pp];
+  pl.K = env_int(precision == F3D_F64 ? "F3D_PAIRSUM_K64" : "F3D_PAIRSUM_K32", k0);
   if (pl.K != 1 && pl.K != 2 && pl.K != 4) pl.K = 2;
   pl.ns_pad = (int)align_up((size_t)(ns > 0 ? ns : 1), kTile);
   const int64_t waves_per_split = ceil_div(nt > 0 ? nt : 1, 64 * pl.K);
-  int target_waves = env_int("F3D_PAIRSUM_WAVES", g_launch_waves[kk][pp]);
+  int target_waves = env_int("F3D_PAIRSUM_WAVES", w0);
   if (target_waves < 64) target_waves = 64;
   const int64_t tiles = pl.ns_pad / kTile;
   int64_t P = ceil_div(target_waves, waves_per_split);
@@ -739,6 +754,41 @@ extern "C" int f3d_pairsum_set_launch(int kernel, int precision, int32_t k,
   const int pp = precision == F3D_F64 ? 0 : 1;
   g_launch_k[kernel][pp] = k ? k : 2;
   g_launch_waves[kernel][pp] = target_waves ? target_waves : 8192;
+  return F3D_OK;
+}
+
+extern "C" int f3d_pairsum_set_launch_bucket(int kernel, int precision, int32_t lg_nt,
+                                             int32_t lg_ns, int32_t k, int32_t target_waves) {
+  F3D_REQUIRE(kernel == F3D_OSEEN || kernel == F3D_ROTLET || kernel == F3D_STRESSLET, F3D_EINVAL,
+              "set_launch_bucket: unknown kernel %d", kernel);
+  F3D_REQUIRE(precision == F3D_F64 || precision == F3D_F32C, F3D_EINVAL,
+              "set_launch_bucket: unknown precision %d", precision);
+  F3D_REQUIRE(lg_nt >= kLgMin && lg_nt <= kLgMax && lg_ns >= kLgMin && lg_ns <= kLgMax,
+              F3D_EINVAL, "set_launch_bucket: log2 sizes (%d, %d) outside [%d, %d]", lg_nt, lg_ns,
+              kLgMin, kLgMax);
+  F3D_REQUIRE(k == 0 || k == 1 || k == 2 || k == 4, F3D_EINVAL,
+              "set_launch_bucket: K must be 0, 1, 2 or 4");
+  F3D_REQUIRE(k == 0 || (target_waves >= 64 && target_waves <= (1 << 20)), F3D_EINVAL,
+              "set_launch_bucket: target_waves outside [64, 2^20]");
+  const int pp = precision == F3D_F64 ? 0 : 1;
+  g_bucket_k[kernel][pp][lg_nt - kLgMin][lg_ns - kLgMin] = k;
+  g_bucket_waves[kernel][pp][lg_nt - kLgMin][lg_ns - kLgMin] = k ? target_waves : 0;
+  return F3D_OK;
+}
+
+extern "C" int f3d_pairsum_plan(int kernel, int precision, int64_t ns, int64_t nt,
+                                int32_t* out) {
+  F3D_REQUIRE(kernel == F3D_OSEEN || kernel == F3D_ROTLET || kernel == F3D_STRESSLET, F3D_EINVAL,
+              "pairsum_plan: unknown kernel %d", kernel);
+  F3D_REQUIRE(precision == F3D_F64 || precision == F3D_F32C, F3D_EINVAL,
+              "pairsum_plan: unknown precision %d", precision);
+  F3D_REQUIRE(ns >= 0 && nt >= 0 && ns < (1ll << 30) && nt < (1ll << 30), F3D_EINVAL,
+              "pairsum_plan: bad sizes");
+  F3D_REQUIRE(out != nullptr, F3D_EINVAL, "pairsum_plan: NULL output");
+  const Plan pl = make_plan(kernel, precision, ns, nt);
+  out[0] = pl.K;
+  out[1] = pl.P;
+  out[2] = pl.chunk;
   return F3D_OK;
 }
 

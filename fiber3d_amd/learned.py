@@ -168,11 +168,16 @@ class LearnedGuess:
     than x by about the preconditioner's quality.  Solutions are accepted
     just below the tolerance, and extrapolating them passes their errors on:
     the plain guess's residual sits near the tolerance.  correct=True
-    extrapolates the corrected history x_k + c_k instead.
+    extrapolates the corrected history x_k + c_k instead.  How much the
+    correction helps depends on how closely the preconditioner resembles the
+    operator: with the block-diagonal shell block of a large periphery it
+    can make the guess worse.
 
     The net (use_net=True): one MLP per fiber node predicts the correction
-    of the baseline guess (the extrapolation of the corrected history when
-    the stepper hands over corrections, else the plain extrapolation).  Its
+    of the baseline guess.  The baseline is the corrected extrapolation when
+    the correction has lowered the residual it was applied to, on average
+    over the last calib_steps sampling steps (the stepper measures the gain
+    |r - A P^-1 r| / |r| in FP64 there), else the plain one.  Its
     target is the corrected solution x + P^-1 r minus that guess: the
     position part in an orthonormal local frame (tangent, the part of the
     last displacement normal to it, their cross product) and the tension
@@ -186,15 +191,15 @@ class LearnedGuess:
     layer starts at zero, so an untrained net reproduces the baseline.  After
     training it must beat the baseline twice on held-out data: its validation
     MSE below `gate` times the baseline's on the last val_frac of the
-    training steps, then its guesses' residuals (FP32-compensated products)
-    below `gate` times the baseline guesses' over the next `shadow_steps`
+    training steps, then its guesses' FP64 residuals below `gate` times the
+    baseline guesses' over the next `shadow_steps`
     steps, during which the baseline still drives the solves.  Other
     components keep the plain extrapolation.
     """
 
     def __init__(self, train_steps=100, epochs=200, lr=1e-3, batch=8192, seed=0,
                  use_net=True, gate=0.9, val_frac=0.2, fit=(6, 2), correct=False,
-                 max_samples=None, modes=8, shadow_steps=4):
+                 max_samples=None, modes=8, shadow_steps=4, calib_steps=4):
         self.fit = (int(fit[0]), int(fit[1]))
         lsq_weights(*self.fit)  # validates the shape
         self.keep = max(3, self.fit[0])
@@ -221,10 +226,24 @@ class LearnedGuess:
         self.enabled_f = False
         self.modes = modes
         self.shadow_steps = shadow_steps
+        # which extrapolation the net corrects: decided at training time by
+        # the correction gains measured on the last calib_steps sampling steps
+        self.calib_steps = calib_steps
+        self.base_mode = None
+        self._calib = []
         self.shadow_left = 0
         self.shadow = []
         self.trained = False
         self.report = {}
+
+    @property
+    def warm_steps(self):
+        """Steps before the predictor's final form: the history, the
+        training samples (the baseline is chosen on the last calib_steps of
+        them) and the gate steps."""
+        if not self.use_net:
+            return self.keep + 2
+        return self.keep + self.train_steps + self.shadow_steps + 2
 
     @property
     def wants_correction(self):
@@ -346,21 +365,23 @@ class LearnedGuess:
             return self._extrapolated(ts, corrected=have_cor and self.correct)
         x_ext = self._extrapolated(ts)
         x_cor = self._extrapolated(ts, corrected=True) if have_cor else None
-        # the net corrects the residual-corrected extrapolation where the
-        # stepper hands over corrections, else the plain one
-        base = x_cor if x_cor is not None else x_ext
         if len(self._rows) < self.keep:
-            return base
+            return x_cor if x_cor is not None else x_ext
         if not self.trained:
-            self._pending = (*self._fiber_terms(ts, x_ext, x_cor), base)
-            return base
+            # sampling: both extrapolations are kept, the corrected one (when
+            # there is one) drives the solve
+            self._pending = (*self._fiber_terms(ts, x_ext, x_cor), x_ext, x_cor)
+            return x_cor if x_cor is not None else x_ext
+        # the net corrects the better of the two extrapolations (the plain one
+        # when the stepper hands over no corrections)
+        base = x_cor if (x_cor is not None and self.base_mode == "corrected") else x_ext
         if self.shadow_left > 0:
             # held-out check of the trained net on the quantity that matters:
-            # its guess's residual against the baseline's (FP32-compensated
-            # products), while the baseline guess still drives the solve
+            # its guess's FP64 residual against the baseline's, while the
+            # baseline guess still drives the solve
             x_net = self._net_guess(ts, base, x_ext, x_cor)
             S = ts.sys
-            rn = lambda x: float(torch.linalg.norm(S.rhs - S.matvec(x, precision="f32c")))
+            rn = lambda x: float(torch.linalg.norm(S.rhs - S.matvec(x)))
             self.shadow.append((rn(x_net), rn(base)))
             self.shadow_left -= 1
             if self.shadow_left == 0:
@@ -406,24 +427,43 @@ class LearnedGuess:
             x0[ts.idx_t] = x0[ts.idx_t] + d[:, 3]
         return x0
 
-    def observe(self, ts, sol, corr=None):
-        """Record an accepted solution and its residual correction P^-1 r."""
+    @property
+    def needs_gain(self):
+        """Whether the stepper should measure the correction's gain
+        |r - A P^-1 r| / |r| on this step (the last calib_steps sampling
+        steps of the net mode)."""
+        return (self.use_net and not self.trained and self._have_corrected()
+                and len(self._rows) >= self.keep
+                and len(self.samples_f) >= self.train_steps - self.calib_steps)
+
+    def observe(self, ts, sol, corr=None, gain=None):
+        """Record an accepted solution, its residual correction P^-1 r and,
+        when measured, the correction's gain |r - A P^-1 r| / |r| (below 1:
+        x + P^-1 r has the smaller residual)."""
+        if gain is not None:
+            self._calib.append(float(gain))
         pend = getattr(self, "_pending", None)
         self._push(sol, corr)
         self.times = (self.times + [float(ts.time)])[-self.keep:]
         if self.trained or pend is None:
             return
-        ff, frame, base = pend
-        # target: the corrected solution x + P^-1 r where given (closer to
-        # the exact solution than x) minus the guess the net corrects
-        d = (sol if corr is None else sol + corr) - base
-        tgt = torch.cat([torch.einsum("nij,nj->ni", frame, d[ts.idx_x]),
-                         d[ts.idx_t][:, None]], dim=1)
+        ff, frame, x_ext, x_cor = pend
+        # targets: the corrected solution x + P^-1 r where given (closer to
+        # the exact solution than x) minus each candidate baseline (the
+        # baseline is chosen at training time)
+        target = sol if corr is None else sol + corr
+
+        def local(base):
+            d = target - base
+            return torch.cat([torch.einsum("nij,nj->ni", frame, d[ts.idx_x]),
+                              d[ts.idx_t][:, None]], dim=1)
+        t_ext = local(x_ext)
+        t_cor = local(x_cor) if x_cor is not None else t_ext
         if self.max_samples is not None and ff.shape[0] > self.max_samples:
             pick = torch.randperm(ff.shape[0], generator=self._rng)[:self.max_samples]
             pick = pick.to(ff.device)
-            ff, tgt = ff[pick], tgt[pick]
-        self.samples_f.append((ff, tgt))
+            ff, t_ext, t_cor = ff[pick], t_ext[pick], t_cor[pick]
+        self.samples_f.append((ff, t_ext, t_cor))
         if len(self.samples_f) >= self.train_steps:
             self.train()
 
@@ -451,7 +491,16 @@ class LearnedGuess:
 
     def train(self):
         torch.manual_seed(self.seed)
-        self.net_f, self.nf, self.f_scale, mn, mb = self._train_one(self.samples_f, 4, self.seed)
+        if self._calib:
+            g = float(np.mean(self._calib))
+            self.base_mode = "corrected" if g < 1.0 else "plain"
+            self.report["baseline"] = {"mode": self.base_mode, "correction_gain": g,
+                                       "steps": len(self._calib)}
+        else:
+            self.base_mode = "plain"
+        col = 2 if self.base_mode == "corrected" else 1
+        samples = [(s[0], s[col]) for s in self.samples_f]
+        self.net_f, self.nf, self.f_scale, mn, mb = self._train_one(samples, 4, self.seed)
         # first gate: the held-out steps' MSE; the second is the held-out
         # guess residual over the next `shadow_steps` steps (predict)
         self.enabled_f = False

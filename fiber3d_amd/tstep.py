@@ -271,10 +271,15 @@ class TimeStepper:
         sol = res.x
         # the residual correction P^-1 r for the predictor, with this step's
         # preconditioner (before the re-assembly below)
-        corr = None
+        corr = gain = None
         if (self.predictor is not None and getattr(self.predictor, "wants_correction", False)
                 and res.resid is not None):
             corr = S.precond(res.resid)
+            if getattr(self.predictor, "needs_gain", False):
+                # |r - A P^-1 r| / |r| with this step's operator (FP64)
+                rr = res.resid
+                gain = float(torch.linalg.norm(rr - S.matvec(corr)) /
+                             torch.linalg.norm(rr).clamp_min(1e-300))
         body_old = S.bodies.state() if S.bodies is not None else None
         if self.record:
             # S.rhs and sol are fresh tensors of this step; X is updated in place
@@ -303,7 +308,10 @@ class TimeStepper:
             self.time += dt
             if self.predictor is not None:
                 if getattr(self.predictor, "wants_correction", False):
-                    self.predictor.observe(self, sol, corr)
+                    if gain is not None:
+                        self.predictor.observe(self, sol, corr, gain)
+                    else:
+                        self.predictor.observe(self, sol, corr)
                 else:
                     self.predictor.observe(self, sol)
             self.prev_solution = sol

@@ -69,6 +69,39 @@ int f3d_abi_version(void);
 int f3d_pairsum_set_launch(int kernel, int precision, int32_t k, int32_t target_waves);
 
 /*
+ * The same for one size bucket: calls with ceil(log2 nt) == lg_nt and
+ * ceil(log2 ns) == lg_ns (both clamped to [8, 22]) use (k, target_waves);
+ * k == 0 returns the bucket to the (kernel, precision) shape above.  The
+ * autotuner fills every bucket from its nearest tuned problem size.  No
+ * reference counterpart.
+ */
+int f3d_pairsum_set_launch_bucket(int kernel, int precision, int32_t lg_nt, int32_t lg_ns,
+                                  int32_t k, int32_t target_waves);
+
+/*
+ * The launch plan f3d_pairsum would use for (kernel, precision, ns, nt):
+ * out[0] = targets per lane, out[1] = source chunks, out[2] = sources per
+ * chunk.  Host-only (no device work); no reference counterpart.
+ */
+int f3d_pairsum_plan(int kernel, int precision, int64_t ns, int64_t nt, int32_t* out);
+
+/*
+ * Launch choices of the batched LU for one precision (F3D_F64 or F3D_F32C):
+ * waves per CU of the one-wave blocked factorisation (1..4 for FP64, 1..8
+ * for FP32; 0 restores the largest), and whether solves without refinement
+ * stage the factor in LDS when it fits in 64 KB (1, the default) or stream
+ * it from HBM (0).  Set by the autotuner; no reference counterpart.
+ */
+int f3d_lu_set_launch(int precision, int32_t factor_waves_per_cu, int32_t solve_staged);
+
+/*
+ * Whether f3d_gemm_bf16 uses its streaming kernel for N == n (128 or 256,
+ * row-major operands, K <= 256): enable 1 (the default) or 0 (the tiled
+ * kernel).  Set by the autotuner; no reference counterpart.
+ */
+int f3d_gemm_set_stream(int32_t n, int32_t enable);
+
+/*
  * Timing of the pair kernel alone (not the bounding-box, pack and reduce
  * launches around it): enable != 0 clears the record and makes every later
  * f3d_pairsum / f3d_periphery_apply record a HIP event pair around its pair

@@ -38,12 +38,13 @@ def test_bad_guess_falls_back_to_zero_start():
 
 
 def test_learned_guess_beats_extrapolation_on_held_out_steps():
-    """A time series: the learned predictor trains its fiber net on the first
-    `train` steps and checks it on 4 gate steps; then all predictors run the
-    same number of further (held-out) steps.  The learned predictor's
-    baseline is the extrapolation of the residual-corrected history
-    (x + P^-1 r); on the held-out steps its guesses have at least 10% smaller
-    residuals and need at least 10% fewer GMRES iterations than the plain
+    """A time series: the learned predictor trains its fiber net on its
+    first `train` steps (picking its baseline on the last 4 of them) and
+    checks it on 4 gate steps; then all predictors run the same number of
+    further (held-out) steps.  Here (dense shell preconditioner) the baseline is the
+    extrapolation of the residual-corrected history (x + P^-1 r); on the
+    held-out steps the learned guesses have at least 10% smaller residuals
+    and need at least 10% fewer GMRES iterations than the plain
     extrapolation, and never do worse than the corrected extrapolation alone:
     the net is kept only if it beats its baseline on both gates (validation
     MSE, then guess residuals, below 0.9 times the baseline's)."""
@@ -57,7 +58,7 @@ def test_learned_guess_beats_extrapolation_on_held_out_steps():
              ("learned", LearnedGuess(train_steps=train, epochs=60)))
     for name, pred in preds:
         ts = TimeStepper(fibers, shell, mode="fast", tol=1e-8, predictor=pred)
-        warm = 8 + train + 4  # the learned predictor's history, training and gate steps
+        warm = preds[-1][1].warm_steps  # the learned predictor's whole warm-up window
         stats = [ts.step() for _ in range(warm + held)]
         assert all(st.residual <= 1e-8 for st in stats)
         tail = stats[warm:]
@@ -73,6 +74,7 @@ def test_learned_guess_beats_extrapolation_on_held_out_steps():
     # the gated net never makes the corrected extrapolation worse (the bar
     # leaves room for the trajectories' atomic-summation differences)
     assert r0s["learned"] <= 1.05 * r0s["corrected"]
+    assert reports["learned"]["baseline"]["mode"] == "corrected"
     rep = reports["learned"]["fiber"]
     if rep["enabled"]:
         assert rep["val_mse_net"] < 0.9 * rep["val_mse_baseline"]
