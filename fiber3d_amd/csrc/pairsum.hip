@@ -323,13 +323,20 @@ pairsum_kernel(const T* __restrict__ rec, int chunk, int ns, int ns_pad,
   const T reg2 = (T)((reg / hs) * (reg / hs));
   const T eps2 = (T)((eps / hs) * (eps / hs));
   const T rmax = (T)(hs / eps);
-  const int base = blockIdx.x * (kBlock * K) + threadIdx.x;
+  // The G = blockDim.x / 64 waves of a block share their 64 K targets and
+  // take consecutive source chunks; their sums are combined in LDS, so the
+  // block writes one partial per target (G times fewer partial bytes).
+  // (readfirstlane: the wave index is wave-uniform, so the source records
+  // stay scalar loads into SGPRs)
+  const int lane = threadIdx.x & 63, G = blockDim.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int base = blockIdx.x * (64 * K) + lane;
   TCoord<T> tx[K], ty[K], tz[K];
   int lo[K], hi[K];
   int wlo = 0x7fffffff, whi = -0x7fffffff;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int i = base + k * kBlock;
+    const int i = base + k * 64;
     const int ic = i < nt ? i : nt - 1;
     const long b = (long)ic * t_sp;
     if constexpr (kQuant) {
@@ -360,7 +367,9 @@ pairsum_kernel(const T* __restrict__ rec, int chunk, int ns, int ns_pad,
 #pragma unroll
   for (int k = 0; k < K; ++k) sx[k] = sy[k] = sz[k] = cx[k] = cy[k] = cz[k] = 0;
 
-  const int j_begin = blockIdx.y * chunk;
+  // this wave's chunk; past the last chunk (the final block's spare waves)
+  // the loop is empty and the wave adds zeros
+  const int j_begin = min((int)(blockIdx.y * G + wv) * chunk, ns_pad);
   const int j_end = min(j_begin + chunk, ns_pad);
   for (int j0 = j_begin; j0 < j_end; j0 += kTile) {
     T ax[K], ay[K], az[K];
@@ -399,18 +408,48 @@ pairsum_kernel(const T* __restrict__ rec, int chunk, int ns, int ns_pad,
       }
     }
   }
+  // each wave's sums in FP64 (FP32: the Kahan pair s - c), the block's
+  // waves added in a fixed order
+  double v[3][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    v[0][k] = (double)sx[k] - (double)cx[k];
+    v[1][k] = (double)sy[k] - (double)cy[k];
+    v[2][k] = (double)sz[k] - (double)cz[k];
+  }
+  __shared__ double red[3][3][K][64];
+  if (wv > 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[wv - 1][c][k][lane] = v[c][k];
+  }
+  __syncthreads();
+  if (wv != 0) return;
+  // (one wave's values at a time: the epilogue must not set the kernel's
+  // register count)
+#pragma unroll 1
+  for (int w = 1; w < G; ++w) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[c][k] += red[w - 1][c][k][lane];
+  }
   const size_t p = blockIdx.y;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int i = base + k * kBlock;
+    const int i = base + k * 64;
     if (i < nt) {
-      part_hi[(p * 3 + 0) * nt + i] = sx[k];
-      part_hi[(p * 3 + 1) * nt + i] = sy[k];
-      part_hi[(p * 3 + 2) * nt + i] = sz[k];
-      if (sizeof(T) == 4) {
-        part_lo[(p * 3 + 0) * nt + i] = -cx[k];
-        part_lo[(p * 3 + 1) * nt + i] = -cy[k];
-        part_lo[(p * 3 + 2) * nt + i] = -cz[k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (sizeof(T) == 4) {
+          // the FP64 sum as an unevaluated FP32 pair (hi + lo)
+          const T h = (T)v[c][k];
+          part_hi[(p * 3 + c) * nt + i] = h;
+          part_lo[(p * 3 + c) * nt + i] = (T)(v[c][k] - (double)h);
+        } else {
+          part_hi[(p * 3 + c) * nt + i] = (T)v[c][k];
+        }
       }
     }
   }
@@ -500,9 +539,16 @@ reduce_kernel(const T* __restrict__ part_hi, const T* __restrict__ part_lo, int 
 struct Plan {
   int K;
   int ns_pad;
-  int P;
-  int chunk;
+  int P;      // source chunks (one wave each per target group)
+  int chunk;  // sources per chunk
+  int G;      // chunks per block (waves of a block, combined in LDS)
+  int Pg;     // partial sums per target: ceil(P / G)
 };
+
+static void set_groups(Plan& pl) {
+  pl.G = pl.P >= 4 ? 4 : (pl.P >= 2 ? 2 : 1);
+  pl.Pg = (int)ceil_div(pl.P, pl.G);
+}
 
 static int env_int(const char* name, int dflt) {
   const char* s = getenv(name);
@@ -558,6 +604,7 @@ static Plan make_plan(int kernel, int precision, int64_t ns, int64_t nt) {
   const int64_t tiles_per = ceil_div(tiles, P);
   pl.chunk = (int)(tiles_per * kTile);
   pl.P = (int)ceil_div(pl.ns_pad, pl.chunk);
+  set_groups(pl);
   return pl;
 }
 
@@ -573,7 +620,7 @@ static size_t part_bytes(int precision, int P, int64_t nt) {
 }
 
 static size_t plan_ws(int kernel, int precision, const Plan& pl, int64_t nt) {
-  return rec_bytes(kernel, precision, pl.ns_pad) + part_bytes(precision, pl.P, nt) +
+  return rec_bytes(kernel, precision, pl.ns_pad) + part_bytes(precision, pl.Pg, nt) +
          align_up(kPackBlocks * sizeof(double), 256) + align_up(sizeof(Geo), 256) +
          align_up(kBoxBlocks * 6 * sizeof(double), 256);
 }
@@ -588,18 +635,22 @@ static Plan make_plan_within(int kernel, int precision, int64_t ns, int64_t nt, 
   Plan pl = make_plan(kernel, precision, ns, nt);
   if (plan_ws(kernel, precision, pl, nt) <= cap) return pl;
   Plan one = pl;
-  one.P = 1;
+  one.Pg = 1;
   const size_t fixed = plan_ws(kernel, precision, one, nt) - part_bytes(precision, 1, nt);
   const size_t per = 3 * (size_t)(nt > 0 ? nt : 1) * 8;
-  int64_t pmax = cap > fixed + 256 ? (int64_t)((cap - fixed - 256) / per) : 0;
-  if (pmax < 1) {
+  // partial groups that fit, each of up to 4 chunks
+  int64_t gmax = cap > fixed + 256 ? (int64_t)((cap - fixed - 256) / per) : 0;
+  if (gmax < 1) {
     pl.P = 0;
     return pl;
   }
   const int64_t tiles = pl.ns_pad / kTile;
+  int64_t pmax = 4 * gmax;
+  if (pmax > pl.P) pmax = pl.P;
   const int64_t tiles_per = ceil_div(tiles, pmax < tiles ? pmax : tiles);
   pl.chunk = (int)(tiles_per * kTile);
   pl.P = (int)ceil_div(pl.ns_pad, pl.chunk);
+  set_groups(pl);
   return pl;
 }
 
@@ -615,8 +666,8 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
   T* rec = reinterpret_cast<T*>(ws);
   char* p = ws + rec_bytes(KIND, prec, pl.ns_pad);
   T* part_hi = reinterpret_cast<T*>(p);
-  T* part_lo = part_hi + (size_t)pl.P * 3 * nt;
-  p += part_bytes(prec, pl.P, nt);
+  T* part_lo = part_hi + (size_t)pl.Pg * 3 * nt;
+  p += part_bytes(prec, pl.Pg, nt);
   double* sigma_part = reinterpret_cast<double*>(p);
   p += align_up(kPackBlocks * sizeof(double), 256);
   Geo* geo = kQuant ? reinterpret_cast<Geo*>(p) : nullptr;
@@ -635,13 +686,14 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
              pos, nrm, den, w, (long)s_sp, (long)s_sc, (const Geo*)geo, rec,
              want_sigma ? sigma_part : nullptr);
 
-  const dim3 grid((unsigned)ceil_div(nt, (int64_t)kBlock * pl.K), (unsigned)pl.P);
+  const dim3 grid((unsigned)ceil_div(nt, (int64_t)64 * pl.K), (unsigned)pl.Pg);
+  const dim3 block((unsigned)(64 * pl.G));
   const bool clamp = kQuant && (flags & F3D_CLAMP_NEAR);
   const int self_excl = (flags & F3D_EXCLUDE_SELF) ? 1 : 0;
   const bool timed = g_timer.on && g_timer.n < kTimerSlots;
   if (timed) F3D_HIP_CHECK(hipEventRecord(g_timer.ev[2 * g_timer.n], st));
 #define F3D_PS(KK, CL)                                                                        \
-  F3D_LAUNCH((pairsum_kernel<T, KIND, KK, CL>), grid, dim3(kBlock), 0, st, rec, pl.chunk,     \
+  F3D_LAUNCH((pairsum_kernel<T, KIND, KK, CL>), grid, block, 0, st, rec, pl.chunk,           \
              (int)ns, pl.ns_pad, tpos, (long)t_sp, (long)t_sc, (int)nt, lo, hi, self_excl,    \
              reg, eps, (const Geo*)geo, part_hi, part_lo)
   if (clamp) {
@@ -668,7 +720,7 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
     epi.sigma_part = sigma_part;
   }
   F3D_LAUNCH((reduce_kernel<T>), dim3((unsigned)ceil_div(nt, (int64_t)kRedTargets)), dim3(256), 0, st, part_hi,
-             part_lo, pl.P, (int)nt, factor, (const Geo*)geo, Degree<KIND>::value, out,
+             part_lo, pl.Pg, (int)nt, factor, (const Geo*)geo, Degree<KIND>::value, out,
              (long)o_sp, (long)o_sc, (flags & F3D_ACCUMULATE) ? 1 : 0, epi);
   return F3D_OK;
 }

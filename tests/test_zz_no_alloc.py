@@ -83,8 +83,9 @@ def test_noalloc_exclusion_and_self_flag(prec):
 
 def test_noalloc_fewer_chunks_when_the_default_plan_does_not_fit():
     # a launch shape aiming at 2^18 waves splits 65536 sources over 262144
-    # targets into 128 chunks: 128 x 3 x 262144 x 8 bytes of partial sums,
-    # more than the static scratch, which then holds fewer, longer chunks
+    # targets into 128 chunks, combined 4 per workgroup: 32 x 3 x 262144 x 8
+    # bytes of partial sums, more than the static scratch, which then holds
+    # fewer, longer chunks
     from fiber3d_amd import autotune
     lib = L.load()
     ns, nt = 65536, 262144
@@ -92,14 +93,14 @@ def test_noalloc_fewer_chunks_when_the_default_plan_does_not_fit():
     s, t, f, n, w = _case(rng, ns, nt)
     try:
         for prec in ("f64", "f32c"):
-            autotune.set_launch("oseen", prec, 2, 1 << 18)
+            autotune.set_launch_bucket("oseen", prec, autotune.lg(nt), autotune.lg(ns), 2,
+                                       1 << 18)
+            assert autotune.plan("oseen", prec, ns, nt)[1] == 128
             assert lib.f3d_pairsum_workspace_size(L.OSEEN, PREC[prec], ns, nt) > (64 << 20)
             ref = _call(False, "oseen", prec, 0, s, f, n, None, t)
             got = _call(True, "oseen", prec, 0, s, f, n, None, t)
             assert _rel(got, ref) <= TOL[prec], (prec, _rel(got, ref))
     finally:
-        for prec in ("f64", "f32c"):
-            autotune.set_launch("oseen", prec, 0, 0)
         autotune.apply()
 
 
