@@ -104,72 +104,113 @@ gemm_bf16_kernel(int M, int N, int K, int kchunk, const float* __restrict__ A, l
 }
 
 // ---------------------------------------------------------------------------
-// Streaming path for the per-node layers (M = nodes, large; N a multiple of
-// 64 up to 256; K <= 256): a workgroup owns 128 rows and ALL N columns, so
-// every activation row is read from HBM once and written once.  Four waves,
-// 2 x 2, each owning 64 rows x N/2 columns as (2 x N/64) 32x32 MFMA tiles
-// (up to 128 accumulator registers).  K is staged through LDS in 32-deep
-// chunks, double buffered: the next chunk's global loads are issued before
-// the current chunk's MFMAs, one barrier per chunk; each 16-deep step
-// reuses every A fragment across the wave's column tiles and every B
-// fragment across its two row tiles.
+// Streaming path for the per-node layers (M = nodes, large; N = 64 NT up to
+// 256; K <= 256): a workgroup owns 128 rows and ALL N columns, so every
+// activation row is read from HBM once and written once.  2 x NT waves, each
+// owning a 64 x 64 block as 2 x 2 32x32 MFMA tiles (64 accumulator
+// registers), so a 256-wide layer runs 8 waves per workgroup.  K is staged
+// through LDS in 32-deep chunks, double
+// buffered: the next chunk's global loads (float4) are issued before the
+// current chunk's MFMAs, one barrier per chunk; each 16-deep step reuses
+// every A fragment across the wave's two column tiles and every B fragment
+// across its two row tiles.
 constexpr int kSM = 128;  // rows per workgroup
 constexpr int kSK = 32;   // K chunk
 constexpr int kSP = kSK + 8;  // LDS row pitch (bf16 elements): 80 B, 16-byte aligned
 
-template <int EPI, int NT>  // NT = N / 64 column tiles per wave (1..2 pairs -> N = 64 NT)
-__global__ void __launch_bounds__(256)
+template <int EPI, int NT>  // NT = N / 64 (2 or 4); 128 NT threads
+__global__ void __launch_bounds__(512)
 gemm_bf16_stream_kernel(int M, int N, int K, const float* __restrict__ A, long sam,
                         const float* __restrict__ B, long sbk, long sbn,
                         float* __restrict__ C, long scm, const float* __restrict__ bias,
                         float* __restrict__ Z, const float* __restrict__ Zin, float alpha) {
-  constexpr int NW = 64 * NT;  // columns of the workgroup = N
+  constexpr int NW = 64 * NT;      // columns of the workgroup = N
+  constexpr int NTH = 128 * NT;    // threads
+  constexpr int AK = kSK / NT;     // A floats per thread and chunk
+  constexpr int BI = kSK / 16;     // B items (k pair, column quad) per thread
   __shared__ __attribute__((aligned(16))) unsigned short As[2][kSM][kSP];
   __shared__ __attribute__((aligned(16))) unsigned short Bt[2][NW][kSP];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / NT, wn = wave % NT;
   const int m0 = blockIdx.x * kSM;
-  f32x16_t acc[2][NT];
+  f32x16_t acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < NT; ++b)
+    for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.0f;
 
-  // staging: A chunk 128 x 32 (16 floats per thread: row tid/2, k half
-  // (tid&1)*16), B chunk 32 x NW (NW*32/256 = 8 NT floats per thread)
-  float ra[16], rb[8 * NT];
+  // staging: A chunk 128 x kSK (row tid / NT, AK consecutive k per thread,
+  // float4 pieces), B chunk kSK x NW as BI items per thread of (k pair, 4
+  // columns): two float4 rows, stored transposed as four (k, k+1) bf16
+  // pairs.  Vector loads where the row is 16-byte aligned and in range.
+  float4 ra[AK / 4];
+  float4 rb[BI][2];
+  const bool bvec = sbn == 1 && ((reinterpret_cast<uintptr_t>(B) | (uintptr_t)(sbk * 4)) & 15) == 0;
+  auto ld4 = [&](const float* __restrict__ p, int avail, bool vec) __attribute__((always_inline)) {
+    if (vec && avail >= 4) return *reinterpret_cast<const float4*>(p);
+    return make_float4(avail > 0 ? p[0] : 0.0f, avail > 1 ? p[1] : 0.0f,
+                       avail > 2 ? p[2] : 0.0f, avail > 3 ? p[3] : 0.0f);
+  };
   auto load = [&](int k0) __attribute__((always_inline)) {
-    const int r = tid >> 1, kh = (tid & 1) * 16;
+    const int r = tid / NT, kh = (tid % NT) * AK;
     const int m = m0 + r;
+    const float* __restrict__ arow = A + (long)(m < M ? m : 0) * sam;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int k = k0 + kh + e;
-      ra[e] = (m < M && k < K) ? A[(long)m * sam + k] : 0.0f;
+    for (int q = 0; q < AK / 4; ++q) {
+      const int k = k0 + kh + 4 * q;
+      const float* __restrict__ pa = arow + k;
+      const bool al = (reinterpret_cast<uintptr_t>(pa) & 15) == 0;
+      ra[q] = m < M ? ld4(pa, K - k, al) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 #pragma unroll
-    for (int e = 0; e < 8 * NT; ++e) {
-      const int idx = tid + 256 * e;  // over (k, n): n fastest
-      const int kk = idx / NW, n = idx - kk * NW;
-      const int k = k0 + kk;
-      rb[e] = (k < K && n < N) ? B[(long)k * sbk + (long)n * sbn] : 0.0f;
+    for (int e = 0; e < BI; ++e) {
+      // over (k pair, column quad), k pairs fastest: consecutive lanes store
+      // consecutive words of one LDS row (no bank conflicts)
+      const int idx = tid + NTH * e;
+      const int kp = idx % (kSK / 2), nq = idx / (kSK / 2);
+      const int k = k0 + 2 * kp, n = 4 * nq;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int kk = k + h;
+        if (kk < K && bvec) {
+          rb[e][h] = ld4(B + (long)kk * sbk + n, N - n, true);
+        } else if (kk < K) {
+          const float* __restrict__ pb = B + (long)kk * sbk + (long)n * sbn;
+          rb[e][h] = make_float4(n < N ? pb[0] : 0.0f, n + 1 < N ? pb[sbn] : 0.0f,
+                                 n + 2 < N ? pb[2 * sbn] : 0.0f, n + 3 < N ? pb[3 * sbn] : 0.0f);
+        } else {
+          rb[e][h] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+      }
     }
   };
+  auto pack2 = [](float lo, float hi) __attribute__((always_inline)) {
+    return (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
+  };
   auto store = [&](int buf) __attribute__((always_inline)) {
-    const int r = tid >> 1, kh = (tid & 1) * 16;
+    const int r = tid / NT, kh = (tid % NT) * AK;
 #pragma unroll
-    for (int e = 0; e < 16; e += 2) {
-      const unsigned v = (unsigned)f2bf(ra[e]) | ((unsigned)f2bf(ra[e + 1]) << 16);
-      *reinterpret_cast<unsigned*>(&As[buf][r][kh + e]) = v;
+    for (int q = 0; q < AK / 8; ++q) {
+      uint4 u;
+      u.x = pack2(ra[2 * q].x, ra[2 * q].y);
+      u.y = pack2(ra[2 * q].z, ra[2 * q].w);
+      u.z = pack2(ra[2 * q + 1].x, ra[2 * q + 1].y);
+      u.w = pack2(ra[2 * q + 1].z, ra[2 * q + 1].w);
+      *reinterpret_cast<uint4*>(&As[buf][r][kh + 8 * q]) = u;
     }
 #pragma unroll
-    for (int e = 0; e < 8 * NT; ++e) {
-      const int idx = tid + 256 * e;
-      const int kk = idx / NW, n = idx - kk * NW;
-      Bt[buf][n][kk] = f2bf(rb[e]);
+    for (int e = 0; e < BI; ++e) {
+      const int idx = tid + NTH * e;
+      const int kp = idx % (kSK / 2), nq = idx / (kSK / 2);
+      const int n = 4 * nq;
+      *reinterpret_cast<unsigned*>(&Bt[buf][n + 0][2 * kp]) = pack2(rb[e][0].x, rb[e][1].x);
+      *reinterpret_cast<unsigned*>(&Bt[buf][n + 1][2 * kp]) = pack2(rb[e][0].y, rb[e][1].y);
+      *reinterpret_cast<unsigned*>(&Bt[buf][n + 2][2 * kp]) = pack2(rb[e][0].z, rb[e][1].z);
+      *reinterpret_cast<unsigned*>(&Bt[buf][n + 3][2 * kp]) = pack2(rb[e][0].w, rb[e][1].w);
     }
   };
   load(0);
@@ -182,19 +223,19 @@ gemm_bf16_stream_kernel(int M, int N, int K, const float* __restrict__ A, long s
 #pragma unroll
     for (int ks = 0; ks < kSK; ks += 16) {
       const int kofs = ks + 8 * (lane >> 5);
-      bf16x8_t fa[2], fb[NT];
+      bf16x8_t fa[2], fb[2];
 #pragma unroll
       for (int a = 0; a < 2; ++a)
         fa[a] = __builtin_bit_cast(
             bf16x8_t, *reinterpret_cast<const uint4*>(&As[buf][64 * wm + 32 * a + (lane & 31)][kofs]));
 #pragma unroll
-      for (int b = 0; b < NT; ++b)
+      for (int b = 0; b < 2; ++b)
         fb[b] = __builtin_bit_cast(
-            bf16x8_t, *reinterpret_cast<const uint4*>(&Bt[buf][NW / 2 * wn + 32 * b + (lane & 31)][kofs]));
+            bf16x8_t, *reinterpret_cast<const uint4*>(&Bt[buf][64 * wn + 32 * b + (lane & 31)][kofs]));
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < NT; ++b)
+        for (int b = 0; b < 2; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
     }
     if (c + 1 < nchunks) store(buf ^ 1);
@@ -204,8 +245,8 @@ gemm_bf16_stream_kernel(int M, int N, int K, const float* __restrict__ A, long s
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < NT; ++b) {
-      const int n = NW / 2 * wn + 32 * b + (lane & 31);
+    for (int b = 0; b < 2; ++b) {
+      const int n = 64 * wn + 32 * b + (lane & 31);
       const float bn = (bias && n < N) ? bias[n] : 0.0f;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
@@ -268,7 +309,7 @@ extern "C" int f3d_gemm_bf16(int epi, int64_t M, int64_t N, int64_t K, int32_t k
                          K <= 256 && M >= kSM;
   if (stream_ok) {
     const dim3 g((unsigned)ceil_div(M, kSM));
-#define F3D_S(E, T)                                                                            F3D_LAUNCH((gemm_bf16_stream_kernel<E, T>), g, dim3(256), 0, st, (int)M, (int)N, (int)K, A,              (long)sam, B, (long)sbk, (long)sbn, C, (long)scm, bias, Z, Zin, alpha)
+#define F3D_S(E, T)                                                                            F3D_LAUNCH((gemm_bf16_stream_kernel<E, T>), g, dim3(128 * T), 0, st, (int)M, (int)N, (int)K, A,              (long)sam, B, (long)sbk, (long)sbn, C, (long)scm, bias, Z, Zin, alpha)
 #define F3D_SE(E) if (N == 256) F3D_S(E, 4); else F3D_S(E, 2)
     switch (epi) {
       case EPI_STORE: F3D_SE(EPI_STORE); break;
