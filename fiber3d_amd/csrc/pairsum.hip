@@ -7,9 +7,10 @@
 //
 // Design (MI355X / CDNA4, wave64):
 //   * pack_kernel converts the caller's sources (any AoS/SoA strides, FP64) into
-//     one contiguous record per source (8 or 12 32/64-bit words: position,
+//     one contiguous record per source (6 or 9 32/64-bit words: position,
 //     density[, normal]), multiplying an optional per-source weight, and
-//     zero-padding to a multiple of the 64-source tile.
+//     zero-padding to a multiple of the 64-source tile.  tile_box_kernel
+//     records each tile's centre and radius.
 //   * pairsum_kernel: each lane owns K targets (registers).  The source index
 //     is wave-uniform, so source records are fetched with scalar loads into
 //     SGPRs and every VALU op takes its source operand straight from an SGPR:
@@ -31,6 +32,11 @@
 //   * Exclusion ranges [lo_i, hi_i) (same fiber / self term) are tested only in
 //     tiles that intersect the wave's union of ranges; all other tiles run a
 //     mask-free path.
+//   * Far tiles: a tile whose centre is more than twice its radius (and more
+//     than its radius + 2 eps) from the box of the wave's targets has no near
+//     pair, so it runs without the vote; FP32C evaluates it on FP32
+//     coordinates relative to the tile's quantised centre (far records),
+//     without the integer subtract and convert.
 #include "common.h"
 
 #include <stdarg.h>
@@ -46,10 +52,33 @@ constexpr int kBlock = 256;      // threads per block (4 waves)
 constexpr int kPackBlocks = 256; // grid of the pack kernel (sigma partials)
 constexpr double kQuantLevels = 2147483648.0 - 65536.0;  // 2^31 minus margin
 
+// Record words per source, unpadded so that a group of records loads with a
+// few wide scalar loads: 6 for Oseen and rotlet (x, y, z, f or tau), 9 for
+// the stresslet (x, y, z, n, q).  FP32C records hold the quantised position
+// as int32 bits in words 0..2; their far records (same layout) hold the
+// position relative to the source tile's quantised centre, in units of the
+// quantum, as FP32.
 template <int KIND>
 struct RecLen {
-  static constexpr int value = (KIND == F3D_STRESSLET) ? 12 : 8;
+  static constexpr int value = (KIND == F3D_STRESSLET) ? 9 : 6;
 };
+
+// Tile geometry for the near/far split: centre and radius of the sources of
+// one 64-source tile (radius < 0: no real source) and, for FP32C, the
+// quantised centre.  A tile is far from a wave when the distance from its
+// centre to the box of the wave's targets exceeds both the far ratio (FP32C;
+// 0 for FP64) times its radius and its radius plus 2 eps: then no pair of
+// the two is near (r > eps: no regularisation, no vote), and for FP32C the
+// tile-relative FP32 coordinates give d with a relative error of a few ulp
+// (|target - centre| and |source - centre| are at most a small multiple of
+// |d|).
+struct TileBox {
+  double c[3];
+  double r;
+  int q[3];
+  int pad;
+};
+constexpr double kFarRatio32 = 2.0;
 
 // homogeneity degree of the kernel in d (for the FP32C rescaling)
 template <int KIND>
@@ -164,6 +193,57 @@ __device__ __forceinline__ int quantise(double x, double origin, double inv_h) {
 }
 
 // ---------------------------------------------------------------------------
+// Tile boxes: one wave per 64-source tile, the bounding box's centre and the
+// largest distance of a real source from it.
+__device__ __forceinline__ double readfirst_f64(double v) {
+  const unsigned long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(u & 0xffffffffu));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
+  return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_min_f64(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__global__ void __launch_bounds__(64)
+tile_box_kernel(int ns, int tiles, const double* __restrict__ pos, long sp, long sc,
+                const Geo* __restrict__ geo, TileBox* __restrict__ box) {
+  const int t = blockIdx.x;
+  if (t >= tiles) return;
+  const int j = t * kTile + (int)threadIdx.x;
+  const bool valid = j < ns;
+  double x[3], lo[3], hi[3], c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    x[a] = valid ? pos[(long)j * sp + a * sc] : 0.0;
+    lo[a] = wave_min_f64(valid ? x[a] : 1e300);
+    hi[a] = wave_max_f64(valid ? x[a] : -1e300);
+    c[a] = 0.5 * (lo[a] + hi[a]);
+  }
+  double r2 = 0.0;
+  if (valid) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r2 += (x[a] - c[a]) * (x[a] - c[a]);
+  }
+  const double rmax = sqrt(wave_max_f64(r2));
+  const bool any = t * kTile < ns;
+  if (threadIdx.x == 0) {
+    TileBox b;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      b.c[a] = any ? c[a] : 0.0;
+      b.q[a] = (any && geo) ? quantise(c[a], geo->origin[a], geo->inv_h) : 0;
+    }
+    // rounding of the centre and the root: a relative margin
+    b.r = any ? rmax * (1.0 + 1e-12) + 1e-300 : -1.0;
+    b.pad = 0;
+    box[t] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // pack: sources -> records.  Also accumulates sigma = sum_j n_j . q_j per
 // block when sigma_part != nullptr (periphery complementary term).
 // FP32C records hold the quantised position as int32 bits in words 0..2.
@@ -171,41 +251,52 @@ template <typename T, int KIND>
 __global__ void __launch_bounds__(256)
 pack_kernel(int ns, int ns_pad, const double* __restrict__ pos, const double* __restrict__ nrm,
             const double* __restrict__ den, const double* __restrict__ w, long sp, long sc,
-            const Geo* __restrict__ geo, T* __restrict__ rec, double* __restrict__ sigma_part) {
+            const Geo* __restrict__ geo, const TileBox* __restrict__ tbox, T* __restrict__ rec,
+            T* __restrict__ rec_far, double* __restrict__ sigma_part) {
   constexpr int R = RecLen<KIND>::value;
   constexpr bool kQuant = std::is_same<T, float>::value;
   double sig = 0.0;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ns_pad; j += gridDim.x * blockDim.x) {
-    T* r = rec + (size_t)j * R;
+    T r[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) r[c] = 0;
+    // far record position words (FP32C)
+    T f[3] = {0, 0, 0};
+    // Zero-density padding for j >= ns; the pair kernel masks those sources,
+    // so a target at the padding position never meets an unregularised r = 0.
     if (j < ns) {
       const double wj = w ? w[j] : 1.0;
       const long b = (long)j * sp;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double x = pos[b + a * sc];
-        if constexpr (kQuant)
-          r[a] = __int_as_float(quantise(x, geo->origin[a], geo->inv_h));
-        else
-          r[a] = x;
-      }
       if (KIND == F3D_STRESSLET) {
         const double n0 = nrm[b], n1 = nrm[b + sc], n2 = nrm[b + 2 * sc];
         const double q0 = den[b] * wj, q1 = den[b + sc] * wj, q2 = den[b + 2 * sc] * wj;
         r[3] = (T)n0; r[4] = (T)n1; r[5] = (T)n2;
         r[6] = (T)q0; r[7] = (T)q1; r[8] = (T)q2;
-        r[9] = 0; r[10] = 0; r[11] = 0;
         sig += n0 * q0 + n1 * q1 + n2 * q2;
       } else {
         r[3] = (T)(den[b] * wj);
         r[4] = (T)(den[b + sc] * wj);
         r[5] = (T)(den[b + 2 * sc] * wj);
-        r[6] = 0; r[7] = 0;
       }
-    } else {
-      // Zero-density padding; the pair kernel masks sources j >= ns, so a
-      // target at the padding position never meets an unregularised r = 0.
 #pragma unroll
-      for (int c = 0; c < R; ++c) r[c] = 0;
+      for (int a = 0; a < 3; ++a) {
+        const double x = pos[b + a * sc];
+        if constexpr (kQuant) {
+          const int q = quantise(x, geo->origin[a], geo->inv_h);
+          r[a] = __int_as_float(q);
+          f[a] = (float)(q - tbox[j / kTile].q[a]);
+        } else {
+          r[a] = x;
+        }
+      }
+    }
+    T* out = rec + (size_t)j * R;
+#pragma unroll
+    for (int c = 0; c < R; ++c) out[c] = r[c];
+    if constexpr (kQuant) {
+      T* of = rec_far + (size_t)j * R;
+#pragma unroll
+      for (int c = 0; c < R; ++c) of[c] = c < 3 ? f[c] : r[c];
     }
   }
   if (sigma_part) {
@@ -224,6 +315,33 @@ using TCoord = typename std::conditional<std::is_same<T, float>::value, int, dou
 
 // near-pair handling of one pair
 enum { MODE_EXACT = 0, MODE_CLAMP = 1, MODE_VOTE = 2 };
+
+// The kernel's contribution for d = (dx, dy, dz), ri = 1/|d| and the
+// source's density words dv (Oseen f; rotlet tau; stresslet n, q).
+template <typename T, int KIND>
+__device__ __forceinline__ void accum(const T* dv, T dx, T dy, T dz, T ri, T& ax, T& ay,
+                                      T& az) {
+  if (KIND == F3D_OSEEN) {
+    const T fd = dx * dv[0] + dy * dv[1] + dz * dv[2];
+    const T s = fd * (ri * ri);
+    ax += ri * (dv[0] + s * dx);
+    ay += ri * (dv[1] + s * dy);
+    az += ri * (dv[2] + s * dz);
+  } else if (KIND == F3D_ROTLET) {
+    const T ri3 = ri * ri * ri;
+    ax += ri3 * (dv[1] * dz - dv[2] * dy);
+    ay += ri3 * (dv[2] * dx - dv[0] * dz);
+    az += ri3 * (dv[0] * dy - dv[1] * dx);
+  } else {
+    const T ri2 = ri * ri;
+    const T dn = dx * dv[0] + dy * dv[1] + dz * dv[2];
+    const T dq = dx * dv[3] + dy * dv[4] + dz * dv[5];
+    const T f0 = (dq * ri2) * (dn * ri2) * ri;
+    ax += f0 * dx;
+    ay += f0 * dy;
+    az += f0 * dz;
+  }
+}
 
 // One pair: accumulate the un-scaled kernel contribution of record r into
 // (ax, ay, az).  MASK: zero the pair when excl is set.
@@ -248,25 +366,55 @@ __device__ __forceinline__ void pair(const T* r, TCoord<T> tx, TCoord<T> ty, TCo
   T ri = rsqrt_t(r2);
   if (MODE == MODE_CLAMP) ri = ri < rmax ? ri : rmax;
   if (MASK) ri = excl ? (T)0 : ri;
-  if (KIND == F3D_OSEEN) {
-    const T fd = dx * r[3] + dy * r[4] + dz * r[5];
-    const T s = fd * (ri * ri);
-    ax += ri * (r[3] + s * dx);
-    ay += ri * (r[4] + s * dy);
-    az += ri * (r[5] + s * dz);
-  } else if (KIND == F3D_ROTLET) {
-    const T ri3 = ri * ri * ri;
-    ax += ri3 * (r[4] * dz - r[5] * dy);
-    ay += ri3 * (r[5] * dx - r[3] * dz);
-    az += ri3 * (r[3] * dy - r[4] * dx);
+  accum<T, KIND>(r + 3, dx, dy, dz, ri, ax, ay, az);
+}
+
+// The first NV words of the U records from source j on (wave-uniform: scalar
+// loads into SGPRs).
+template <typename T, int R, int NV, int U>
+__device__ __forceinline__ void load_recs(const T* __restrict__ rec, int j, T (&s)[U][NV]) {
+  // one base address and constant offsets, so that adjacent words merge into
+  // wide scalar loads
+  const T* __restrict__ r = rec + (size_t)j * R;
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) s[u][c] = r[u * R + c];
+}
+
+// Records fetched together per group.  FP32 Oseen and rotlet load the next
+// group's records while the current group computes; the FP64 records, and
+// the FP32 stresslet records, of two groups do not fit the scalar registers
+// beside the kernel's own.
+constexpr int kGroup = 4;
+template <typename T, int KIND>
+struct Prefetch {
+  static constexpr bool value = std::is_same<T, float>::value && KIND != F3D_STRESSLET;
+};
+
+// body(s, jj) for each group of U records of the 64-source tile at j0.
+template <typename T, int KIND, int U, class Body>
+__device__ __forceinline__ void for_groups(const T* __restrict__ rec, int j0, Body&& body) {
+  constexpr int R = RecLen<KIND>::value, NV = R;
+  if constexpr (Prefetch<T, KIND>::value) {
+    T s[U][NV], nx[U][NV];
+    load_recs<T, R, NV, U>(rec, j0, s);
+    for (int jj = 0; jj < kTile; jj += U) {
+      // (the last group reloads itself: no read past the tile)
+      load_recs<T, R, NV, U>(rec, j0 + (jj + U < kTile ? jj + U : jj), nx);
+      body(s, jj);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) s[u][c] = nx[u][c];
+    }
   } else {
-    const T ri2 = ri * ri;
-    const T dn = dx * r[3] + dy * r[4] + dz * r[5];
-    const T dq = dx * r[6] + dy * r[7] + dz * r[8];
-    const T f0 = (dq * ri2) * (dn * ri2) * ri;
-    ax += f0 * dx;
-    ay += f0 * dy;
-    az += f0 * dz;
+    for (int jj = 0; jj < kTile; jj += U) {
+      // the U scalar record loads issue before any use, so their latencies overlap
+      T s[U][NV];
+      load_recs<T, R, NV, U>(rec, j0 + jj, s);
+      body(s, jj);
+    }
   }
 }
 
@@ -280,21 +428,11 @@ __device__ __forceinline__ bool tile(const T* __restrict__ rec, int j0, int ns,
                                      const TCoord<T> (&ty)[K], const TCoord<T> (&tz)[K],
                                      const int (&lo)[K], const int (&hi)[K], T reg2, T eps2,
                                      T rmax, T (&ax)[K], T (&ay)[K], T (&az)[K]) {
-  constexpr int R = RecLen<KIND>::value;
-  constexpr int U = 4;  // sources whose records are fetched together
-  constexpr int NV = (KIND == F3D_STRESSLET) ? 9 : 6;
+  constexpr int NV = RecLen<KIND>::value;
   unsigned long long vote = 0;
-  for (int jj = 0; jj < kTile; jj += U) {
-    // Issue the U scalar record loads before any use so their latencies overlap.
-    T s[U][NV];
+  for_groups<T, KIND, kGroup>(rec, j0, [&](const T (&s)[kGroup][NV], int jj) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const T* __restrict__ r = rec + (size_t)(j0 + jj + u) * R;
-#pragma unroll
-      for (int c = 0; c < NV; ++c) s[u][c] = r[c];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < kGroup; ++u) {
       const int j = j0 + jj + u;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -304,19 +442,63 @@ __device__ __forceinline__ bool tile(const T* __restrict__ rec, int j0, int ns,
                                   ay[k], az[k], vote);
       }
     }
-  }
+  });
   return MODE == MODE_VOTE && vote != 0ull;
+}
+
+// One far 64-source tile (no near pair, no exclusion): d from plain
+// coordinates, no vote and no regularisation.  rec: FP64 records, or FP32C
+// far records (positions relative to the tile's quantised centre); lx, ly,
+// lz: the target in the same coordinates.
+template <typename T, int KIND, int K>
+__device__ __forceinline__ void tile_far(const T* __restrict__ rec, int j0, const T (&lx)[K],
+                                         const T (&ly)[K], const T (&lz)[K], T (&ax)[K],
+                                         T (&ay)[K], T (&az)[K]) {
+  constexpr int NV = RecLen<KIND>::value;
+  for_groups<T, KIND, kGroup>(rec, j0, [&](const T (&s)[kGroup][NV], int) {
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const T dx = lx[k] - s[u][0], dy = ly[k] - s[u][1], dz = lz[k] - s[u][2];
+        const T ri = rsqrt_t(dx * dx + dy * dy + dz * dz);
+        accum<T, KIND>(s[u] + 3, dx, dy, dz, ri, ax[k], ay[k], az[k]);
+      }
+    }
+  });
+}
+
+// Far flags of the 64 tiles from tile t0 on (bit l: tile t0 + l, tiles
+// below t_end only), wave-uniform.  Lane l tests tile t0 + l against the
+// wave's target box [bl, bh].
+__device__ __forceinline__ unsigned long long far_mask(const TileBox* __restrict__ tbox, int t0,
+                                                       int t_end, const double (&bl)[3],
+                                                       const double (&bh)[3], double far_ratio,
+                                                       double eps) {
+  const int t = t0 + (int)(threadIdx.x & 63);
+  bool far = false;
+  if (t < t_end) {
+    const TileBox* __restrict__ tb = tbox + t;
+    const double c0 = tb->c[0], c1 = tb->c[1], c2 = tb->c[2], rr = tb->r;
+    const double e0 = fmax(0.0, fmax(bl[0] - c0, c0 - bh[0]));
+    const double e1 = fmax(0.0, fmax(bl[1] - c1, c1 - bh[1]));
+    const double e2 = fmax(0.0, fmax(bl[2] - c2, c2 - bh[2]));
+    const double need = fmax(far_ratio * rr, rr + 2.0 * eps);
+    far = rr >= 0.0 && e0 * e0 + e1 * e1 + e2 * e2 > need * need;
+  }
+  return __ballot(far);
 }
 
 // partial layout: part[(p*3 + c)*nt + i]; for T=float a second array holds
 // the negated Kahan compensation.
 template <typename T, int KIND, int K, bool CLAMP>
 __global__ void __launch_bounds__(kBlock)
-pairsum_kernel(const T* __restrict__ rec, int chunk, int ns, int ns_pad,
-               const double* __restrict__ tpos, long t_sp, long t_sc, int nt,
+pairsum_kernel(const T* __restrict__ rec, const T* __restrict__ rec_far, int chunk, int ns,
+               int ns_pad, const double* __restrict__ tpos, long t_sp, long t_sc, int nt,
                const int* __restrict__ excl_lo, const int* __restrict__ excl_hi, int self_excl,
-               double reg, double eps,
-               const Geo* __restrict__ geo, T* __restrict__ part_hi, T* __restrict__ part_lo) {
+               double reg, double eps, const Geo* __restrict__ geo,
+               const TileBox* __restrict__ tbox, double far_ratio, T* __restrict__ part_hi,
+               T* __restrict__ part_lo) {
   constexpr bool kQuant = std::is_same<T, float>::value;
   // regularisation constants in the units of the coordinates used
   const double hs = kQuant ? geo->h : 1.0;
@@ -334,19 +516,26 @@ pairsum_kernel(const T* __restrict__ rec, int chunk, int ns, int ns_pad,
   TCoord<T> tx[K], ty[K], tz[K];
   int lo[K], hi[K];
   int wlo = 0x7fffffff, whi = -0x7fffffff;
+  double bl[3] = {1e300, 1e300, 1e300}, bh[3] = {-1e300, -1e300, -1e300};
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = base + k * 64;
     const int ic = i < nt ? i : nt - 1;
     const long b = (long)ic * t_sp;
+    const double px = tpos[b], py = tpos[b + t_sc], pz = tpos[b + 2 * t_sc];
+    if (i < nt) {
+      bl[0] = fmin(bl[0], px); bh[0] = fmax(bh[0], px);
+      bl[1] = fmin(bl[1], py); bh[1] = fmax(bh[1], py);
+      bl[2] = fmin(bl[2], pz); bh[2] = fmax(bh[2], pz);
+    }
     if constexpr (kQuant) {
-      tx[k] = quantise(tpos[b], geo->origin[0], geo->inv_h);
-      ty[k] = quantise(tpos[b + t_sc], geo->origin[1], geo->inv_h);
-      tz[k] = quantise(tpos[b + 2 * t_sc], geo->origin[2], geo->inv_h);
+      tx[k] = quantise(px, geo->origin[0], geo->inv_h);
+      ty[k] = quantise(py, geo->origin[1], geo->inv_h);
+      tz[k] = quantise(pz, geo->origin[2], geo->inv_h);
     } else {
-      tx[k] = tpos[b];
-      ty[k] = tpos[b + t_sc];
-      tz[k] = tpos[b + 2 * t_sc];
+      tx[k] = px;
+      ty[k] = py;
+      tz[k] = pz;
     }
     lo[k] = 0;
     hi[k] = 0;
@@ -362,6 +551,12 @@ pairsum_kernel(const T* __restrict__ rec, int chunk, int ns, int ns_pad,
   // Wave-uniform union of the exclusion ranges (kept in SGPRs).
   wlo = __builtin_amdgcn_readfirstlane(wave_min_i32(wlo));
   whi = __builtin_amdgcn_readfirstlane(wave_max_i32(whi));
+  // Wave-uniform box of the wave's targets (FP64).
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    bl[a] = readfirst_f64(wave_min_f64(bl[a]));
+    bh[a] = readfirst_f64(wave_max_f64(bh[a]));
+  }
 
   T sx[K], sy[K], sz[K], cx[K], cy[K], cz[K];
 #pragma unroll
@@ -371,11 +566,39 @@ pairsum_kernel(const T* __restrict__ rec, int chunk, int ns, int ns_pad,
   // the loop is empty and the wave adds zeros
   const int j_begin = min((int)(blockIdx.y * G + wv) * chunk, ns_pad);
   const int j_end = min(j_begin + chunk, ns_pad);
+  unsigned long long fmask = 0;
   for (int j0 = j_begin; j0 < j_end; j0 += kTile) {
     T ax[K], ay[K], az[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = 0;
-    if ((j0 < whi && j0 + kTile > wlo) || j0 + kTile > ns) {
+    // far flags, 64 tiles at a time
+    if (((j0 - j_begin) & (64 * kTile - 1)) == 0)
+      fmask = far_mask(tbox, j0 / kTile, j_end / kTile, bl, bh, far_ratio, eps);
+    const bool masked = (j0 < whi && j0 + kTile > wlo) || j0 + kTile > ns;
+    const bool far = !masked && (fmask & 1ull);
+    fmask >>= 1;
+    if (far) {
+      T lx[K], ly[K], lz[K];
+      if constexpr (kQuant) {
+        const TileBox* __restrict__ tb = tbox + (j0 / kTile);
+        const int q0 = tb->q[0], q1 = tb->q[1], q2 = tb->q[2];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          lx[k] = (T)(tx[k] - q0);
+          ly[k] = (T)(ty[k] - q1);
+          lz[k] = (T)(tz[k] - q2);
+        }
+        tile_far<T, KIND, K>(rec_far, j0, lx, ly, lz, ax, ay, az);
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          lx[k] = tx[k];
+          ly[k] = ty[k];
+          lz[k] = tz[k];
+        }
+        tile_far<T, KIND, K>(rec, j0, lx, ly, lz, ax, ay, az);
+      }
+    } else if (masked) {
       if (CLAMP)
         tile<T, KIND, K, MODE_CLAMP, true>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax,
                                            ay, az);
@@ -608,10 +831,16 @@ static Plan make_plan(int kernel, int precision, int64_t ns, int64_t nt) {
   return pl;
 }
 
+// source records; FP32C adds the far records (same size)
 static size_t rec_bytes(int kernel, int precision, int ns_pad) {
-  const size_t r = kernel == F3D_STRESSLET ? 12 : 8;
+  const size_t r = kernel == F3D_STRESSLET ? RecLen<F3D_STRESSLET>::value : RecLen<F3D_OSEEN>::value;
   const size_t t = precision == F3D_F64 ? 8 : 4;
-  return align_up((size_t)ns_pad * r * t, 256);
+  const size_t one = align_up((size_t)ns_pad * r * t, 256);
+  return precision == F3D_F64 ? one : 2 * one;
+}
+
+static size_t tile_bytes(int ns_pad) {
+  return align_up((size_t)(ns_pad / kTile) * sizeof(TileBox), 256);
 }
 
 static size_t part_bytes(int precision, int P, int64_t nt) {
@@ -620,7 +849,8 @@ static size_t part_bytes(int precision, int P, int64_t nt) {
 }
 
 static size_t plan_ws(int kernel, int precision, const Plan& pl, int64_t nt) {
-  return rec_bytes(kernel, precision, pl.ns_pad) + part_bytes(precision, pl.Pg, nt) +
+  return rec_bytes(kernel, precision, pl.ns_pad) + tile_bytes(pl.ns_pad) +
+         part_bytes(precision, pl.Pg, nt) +
          align_up(kPackBlocks * sizeof(double), 256) + align_up(sizeof(Geo), 256) +
          align_up(kBoxBlocks * 6 * sizeof(double), 256);
 }
@@ -664,7 +894,10 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
   constexpr bool kQuant = std::is_same<T, float>::value;
   const int prec = kQuant ? F3D_F32C : F3D_F64;
   T* rec = reinterpret_cast<T*>(ws);
+  T* rec_far = kQuant ? reinterpret_cast<T*>(ws + rec_bytes(KIND, prec, pl.ns_pad) / 2) : nullptr;
   char* p = ws + rec_bytes(KIND, prec, pl.ns_pad);
+  TileBox* tbox = reinterpret_cast<TileBox*>(p);
+  p += tile_bytes(pl.ns_pad);
   T* part_hi = reinterpret_cast<T*>(p);
   T* part_lo = part_hi + (size_t)pl.Pg * 3 * nt;
   p += part_bytes(prec, pl.Pg, nt);
@@ -682,20 +915,28 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
                min_extent, geo);
   }
   const bool want_sigma = epi_in != nullptr;
+  const int tiles = pl.ns_pad / kTile;
+  F3D_LAUNCH(tile_box_kernel, dim3((unsigned)tiles), dim3(64), 0, st, (int)ns, tiles, pos,
+             (long)s_sp, (long)s_sc, (const Geo*)geo, tbox);
   F3D_LAUNCH((pack_kernel<T, KIND>), dim3(kPackBlocks), dim3(256), 0, st, (int)ns, pl.ns_pad,
-             pos, nrm, den, w, (long)s_sp, (long)s_sc, (const Geo*)geo, rec,
-             want_sigma ? sigma_part : nullptr);
+             pos, nrm, den, w, (long)s_sp, (long)s_sc, (const Geo*)geo,
+             (const TileBox*)tbox, rec, rec_far, want_sigma ? sigma_part : nullptr);
 
   const dim3 grid((unsigned)ceil_div(nt, (int64_t)64 * pl.K), (unsigned)pl.Pg);
   const dim3 block((unsigned)(64 * pl.G));
   const bool clamp = kQuant && (flags & F3D_CLAMP_NEAR);
   const int self_excl = (flags & F3D_EXCLUDE_SELF) ? 1 : 0;
+  // F3D_PAIRSUM_FAR_RATIO overrides the FP32C far ratio (a very large value
+  // turns the near/far split off)
+  const char* fr = getenv("F3D_PAIRSUM_FAR_RATIO");
+  const double far_ratio = kQuant ? ((fr && *fr) ? atof(fr) : kFarRatio32) : 0.0;
   const bool timed = g_timer.on && g_timer.n < kTimerSlots;
   if (timed) F3D_HIP_CHECK(hipEventRecord(g_timer.ev[2 * g_timer.n], st));
 #define F3D_PS(KK, CL)                                                                        \
-  F3D_LAUNCH((pairsum_kernel<T, KIND, KK, CL>), grid, block, 0, st, rec, pl.chunk,           \
-             (int)ns, pl.ns_pad, tpos, (long)t_sp, (long)t_sc, (int)nt, lo, hi, self_excl,    \
-             reg, eps, (const Geo*)geo, part_hi, part_lo)
+  F3D_LAUNCH((pairsum_kernel<T, KIND, KK, CL>), grid, block, 0, st, rec, (const T*)rec_far,  \
+             pl.chunk, (int)ns, pl.ns_pad, tpos, (long)t_sp, (long)t_sc, (int)nt, lo, hi,     \
+             self_excl, reg, eps, (const Geo*)geo, (const TileBox*)tbox, far_ratio, part_hi,  \
+             part_lo)
   if (clamp) {
     if (pl.K == 1) F3D_PS(1, true);
     else if (pl.K == 2) F3D_PS(2, true);
