@@ -518,21 +518,21 @@ def main():
     F = torch.from_numpy(f).to(dev)
     NRM = torch.from_numpy(nrm).to(dev)
     Q = torch.from_numpy(mu * wq[:, None]).to(dev)
-    idx = torch.arange(N_PTS, dtype=torch.int32, device=dev)
-    excl = (idx, idx + 1)
     out = torch.empty((N_PTS, 3), dtype=torch.float64, device=dev)
     pairs = float(N_PTS) * N_PTS
 
     def run(kernel, precision, steps, warmup, nrm_t=None, den=F):
         for _ in range(warmup):
-            FK.pairsum(kernel, X, den, X, src_nrm=nrm_t, excl=excl, precision=precision, out=out)
+            FK.pairsum(kernel, X, den, X, src_nrm=nrm_t, exclude_self=True, precision=precision,
+                       out=out)
         torch.cuda.synchronize()
         # HIP events around the pair kernel alone, recorded by libf3d on the
         # stream it launches on (the caller's current stream)
         FK.kernel_timer(True)
         t0 = time.perf_counter()
         for _ in range(steps):
-            FK.pairsum(kernel, X, den, X, src_nrm=nrm_t, excl=excl, precision=precision, out=out)
+            FK.pairsum(kernel, X, den, X, src_nrm=nrm_t, exclude_self=True, precision=precision,
+                       out=out)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         kms, launches = FK.kernel_timer_read()

@@ -32,6 +32,9 @@
 //   * Exclusion ranges [lo_i, hi_i) (same fiber / self term) are tested only in
 //     tiles that intersect the wave's union of ranges; all other tiles run a
 //     mask-free path.
+//   * Spatial order: FP32C Oseen and rotlet calls of at least 65536 sources
+//     and targets, without exclusion ranges, visit both in Morton order
+//     (sorted on the device each call), so that tiles and waves are compact.
 //   * Far tiles: a tile whose centre is more than twice its radius (and more
 //     than its radius + 2 eps) from the box of the wave's targets has no near
 //     pair, so it runs without the vote; FP32C evaluates it on FP32
@@ -207,17 +210,211 @@ __device__ __forceinline__ double wave_min_f64(double v) {
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// Spatial order (FP32C Oseen and rotlet, large calls): sources and targets
+// are visited in the Morton order of their quantised positions, 5 bits per
+// axis (32^3 cells, points of one cell in their original order), so that the
+// 64 sources of a tile and the 64 K targets of a wave are spatially compact
+// and most tiles are far from most waves.  A stable LSD radix sort of the
+// 15-bit codes in two 8-bit passes: per-block digit counts, one exclusive
+// scan, a stable scatter.  Deterministic (no global atomics), and every
+// buffer it reads was written earlier in the same call (no memset).
+constexpr int kSortBits = 5;          // per axis
+constexpr int kSortBlock = 256;       // threads per sort block
+constexpr int kSortMaxBlocks = 256;   // blocks per pass (a multiple of 16)
+constexpr int kSortScan = 1024;       // threads of the scan
+constexpr int64_t kSortMin = 65536;   // sources and targets of a call that sorts
+
+// 5 bits -> every third bit
+__device__ __forceinline__ unsigned spread3(unsigned v) {
+  v &= 0x1fu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ void __launch_bounds__(256)
+morton_kernel(int n, const double* __restrict__ pos, long sp, long sc,
+              const Geo* __restrict__ geo, unsigned* __restrict__ key, int* __restrict__ val) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    unsigned c = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int q = quantise(pos[(long)i * sp + a * sc], geo->origin[a], geo->inv_h);
+      c |= spread3((unsigned)q >> (31 - kSortBits)) << a;
+    }
+    key[i] = c;
+    val[i] = i;
+  }
+}
+
+// counts[d * gridDim.x + b]: digit d in block b's chunk (blocks past n count 0)
+__global__ void __launch_bounds__(kSortBlock)
+radix_count_kernel(int n, int chunk, int shift, const unsigned* __restrict__ key,
+                   int* __restrict__ counts) {
+  __shared__ int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const long b0 = (long)blockIdx.x * chunk;
+  const long b1 = b0 + chunk < n ? b0 + chunk : n;
+  for (long e = b0 + threadIdx.x; e < b1; e += kSortBlock)
+    atomicAdd(&h[(key[e] >> shift) & 255u], 1);
+  __syncthreads();
+  counts[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of m = 256 * nb counts in place; each thread owns a
+// contiguous run of m / kSortScan (a multiple of 4) counts
+__global__ void __launch_bounds__(kSortScan)
+radix_scan_kernel(int m, int* __restrict__ counts) {
+  __shared__ int part[kSortScan];
+  const int per = m / kSortScan;
+  int4* __restrict__ c4 = reinterpret_cast<int4*>(counts + (long)threadIdx.x * per);
+  int s = 0;
+  for (int e = 0; e < per / 4; ++e) {
+    const int4 v = c4[e];
+    s += v.x + v.y + v.z + v.w;
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < kSortScan; o <<= 1) {
+    const int v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (int e = 0; e < per / 4; ++e) {
+    const int4 v = c4[e];
+    int4 o;
+    o.x = run; run += v.x;
+    o.y = run; run += v.y;
+    o.z = run; run += v.z;
+    o.w = run; run += v.w;
+    c4[e] = o;
+  }
+}
+
+// Stable scatter of block b's chunk, 256 elements at a time: an element's
+// place is its digit's offset for the block, plus the elements of the same
+// digit in earlier rounds, earlier waves and earlier lanes.
+__global__ void __launch_bounds__(kSortBlock)
+radix_scatter_kernel(int n, int chunk, int shift, const int* __restrict__ offs,
+                     const unsigned* __restrict__ kin, const int* __restrict__ vin,
+                     unsigned* __restrict__ kout, int* __restrict__ vout) {
+  __shared__ int base[256];
+  __shared__ int wc[kSortBlock / 64][256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  base[tid] = offs[tid * gridDim.x + blockIdx.x];
+  const long b0 = (long)blockIdx.x * chunk;
+  const long b1 = b0 + chunk < n ? b0 + chunk : n;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (long t0 = b0; t0 < b1; t0 += kSortBlock) {
+#pragma unroll
+    for (int q = 0; q < kSortBlock / 64; ++q) wc[q][tid] = 0;
+    __syncthreads();
+    const long e = t0 + tid;
+    const bool valid = e < b1;
+    unsigned k = 0;
+    int v = 0, d = 0;
+    if (valid) {
+      k = kin[e];
+      v = vin[e];
+      d = (int)((k >> shift) & 255u);
+    }
+    // lanes of this wave holding the same digit
+    unsigned long long m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1;
+      const unsigned long long bb = __ballot(bit);
+      m &= bit ? bb : ~bb;
+    }
+    const int rank = __popcll(m & below);
+    if (valid && rank == 0) wc[w][d] = __popcll(m);
+    __syncthreads();
+    if (valid) {
+      int at = base[d] + rank;
+      for (int q = 0; q < w; ++q) at += wc[q][d];
+      kout[at] = k;
+      vout[at] = v;
+    }
+    __syncthreads();
+    int add = 0;
+#pragma unroll
+    for (int q = 0; q < kSortBlock / 64; ++q) add += wc[q][tid];
+    base[tid] += add;
+    __syncthreads();
+  }
+}
+
+struct SortBufs {
+  unsigned* key[2];
+  int* val[2];
+};
+
+static int sort_blocks(int64_t n, int* chunk) {
+  int64_t c = ceil_div(n > 0 ? n : 1, (int64_t)kSortMaxBlocks);
+  c = (int64_t)align_up((size_t)c, kSortBlock);
+  if (c < 4 * kSortBlock) c = 4 * kSortBlock;
+  *chunk = (int)c;
+  return kSortMaxBlocks;  // blocks past n count zeros
+}
+
+static size_t sort_set_bytes(int64_t n) {
+  return 4 * align_up((size_t)(n > 0 ? n : 1) * 4, 256);
+}
+static size_t sort_count_bytes() { return align_up((size_t)256 * kSortMaxBlocks * 4, 256); }
+
+static SortBufs sort_set(char* p, int64_t n) {
+  const size_t one = align_up((size_t)(n > 0 ? n : 1) * 4, 256);
+  SortBufs b;
+  b.key[0] = reinterpret_cast<unsigned*>(p);
+  b.key[1] = reinterpret_cast<unsigned*>(p + one);
+  b.val[0] = reinterpret_cast<int*>(p + 2 * one);
+  b.val[1] = reinterpret_cast<int*>(p + 3 * one);
+  return b;
+}
+
+// Morton order of n points: *perm = the permutation (sorted place -> index).
+static int spatial_order(int64_t n, const double* pos, int64_t sp, int64_t sc, const Geo* geo,
+                         SortBufs b, int* counts, hipStream_t st, const int** perm) {
+  int chunk;
+  const int nb = sort_blocks(n, &chunk);
+  F3D_LAUNCH(morton_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n, (int64_t)256), 1024)),
+             dim3(256), 0, st, (int)n, pos, (long)sp, (long)sc, geo, b.key[0], b.val[0]);
+  int cur = 0;
+  for (int shift = 0; shift < 3 * kSortBits; shift += 8) {
+    F3D_LAUNCH(radix_count_kernel, dim3(nb), dim3(kSortBlock), 0, st, (int)n, chunk, shift,
+               (const unsigned*)b.key[cur], counts);
+    F3D_LAUNCH(radix_scan_kernel, dim3(1), dim3(kSortScan), 0, st, 256 * nb, counts);
+    F3D_LAUNCH(radix_scatter_kernel, dim3(nb), dim3(kSortBlock), 0, st, (int)n, chunk, shift,
+               (const int*)counts, (const unsigned*)b.key[cur], (const int*)b.val[cur],
+               b.key[cur ^ 1], b.val[cur ^ 1]);
+    cur ^= 1;
+  }
+  *perm = b.val[cur];
+  return F3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tile boxes: one wave per 64-source tile (sources in the order of perm when
+// given), the bounding box's centre and the largest distance of a real
+// source from it.
 __global__ void __launch_bounds__(64)
 tile_box_kernel(int ns, int tiles, const double* __restrict__ pos, long sp, long sc,
-                const Geo* __restrict__ geo, TileBox* __restrict__ box) {
+                const Geo* __restrict__ geo, const int* __restrict__ perm,
+                TileBox* __restrict__ box) {
   const int t = blockIdx.x;
   if (t >= tiles) return;
   const int j = t * kTile + (int)threadIdx.x;
   const bool valid = j < ns;
+  const int o = valid ? (perm ? perm[j] : j) : 0;
   double x[3], lo[3], hi[3], c[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    x[a] = valid ? pos[(long)j * sp + a * sc] : 0.0;
+    x[a] = valid ? pos[(long)o * sp + a * sc] : 0.0;
     lo[a] = wave_min_f64(valid ? x[a] : 1e300);
     hi[a] = wave_max_f64(valid ? x[a] : -1e300);
     c[a] = 0.5 * (lo[a] + hi[a]);
@@ -251,8 +448,9 @@ template <typename T, int KIND>
 __global__ void __launch_bounds__(256)
 pack_kernel(int ns, int ns_pad, const double* __restrict__ pos, const double* __restrict__ nrm,
             const double* __restrict__ den, const double* __restrict__ w, long sp, long sc,
-            const Geo* __restrict__ geo, const TileBox* __restrict__ tbox, T* __restrict__ rec,
-            T* __restrict__ rec_far, double* __restrict__ sigma_part) {
+            const Geo* __restrict__ geo, const TileBox* __restrict__ tbox,
+            const int* __restrict__ perm, T* __restrict__ rec, T* __restrict__ rec_far,
+            double* __restrict__ sigma_part) {
   constexpr int R = RecLen<KIND>::value;
   constexpr bool kQuant = std::is_same<T, float>::value;
   double sig = 0.0;
@@ -265,9 +463,10 @@ pack_kernel(int ns, int ns_pad, const double* __restrict__ pos, const double* __
     // Zero-density padding for j >= ns; the pair kernel masks those sources,
     // so a target at the padding position never meets an unregularised r = 0.
     if (j < ns) {
-      const double wj = w ? w[j] : 1.0;
-      const long b = (long)j * sp;
-      if (KIND == F3D_STRESSLET) {
+      const int o = perm ? perm[j] : j;  // original source index
+      const double wj = w ? w[o] : 1.0;
+      const long b = (long)o * sp;
+      if constexpr (KIND == F3D_STRESSLET) {
         const double n0 = nrm[b], n1 = nrm[b + sc], n2 = nrm[b + 2 * sc];
         const double q0 = den[b] * wj, q1 = den[b + sc] * wj, q2 = den[b + 2 * sc] * wj;
         r[3] = (T)n0; r[4] = (T)n1; r[5] = (T)n2;
@@ -468,7 +667,7 @@ __device__ __forceinline__ void tile_far(const T* __restrict__ rec, int j0, cons
   });
 }
 
-// Far flags of the 64 tiles from tile t0 on (bit l: tile t0 + l, tiles
+// Far flags of the 64 tiles from tile t0 on (bit l: tile t0 + l; tiles
 // below t_end only), wave-uniform.  Lane l tests tile t0 + l against the
 // wave's target box [bl, bh].
 __device__ __forceinline__ unsigned long long far_mask(const TileBox* __restrict__ tbox, int t0,
@@ -493,7 +692,8 @@ __device__ __forceinline__ unsigned long long far_mask(const TileBox* __restrict
 // the negated Kahan compensation.
 template <typename T, int KIND, int K, bool CLAMP>
 __global__ void __launch_bounds__(kBlock)
-pairsum_kernel(const T* __restrict__ rec, const T* __restrict__ rec_far, int chunk, int ns,
+pairsum_kernel(const T* __restrict__ rec, const T* __restrict__ rec_far,
+               const int* __restrict__ tperm, int chunk, int ns,
                int ns_pad, const double* __restrict__ tpos, long t_sp, long t_sc, int nt,
                const int* __restrict__ excl_lo, const int* __restrict__ excl_hi, int self_excl,
                double reg, double eps, const Geo* __restrict__ geo,
@@ -519,11 +719,15 @@ pairsum_kernel(const T* __restrict__ rec, const T* __restrict__ rec_far, int chu
   double bl[3] = {1e300, 1e300, 1e300}, bh[3] = {-1e300, -1e300, -1e300};
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int i = base + k * 64;
-    const int ic = i < nt ? i : nt - 1;
-    const long b = (long)ic * t_sp;
+    // slot base + k 64 (the partial's index) holds target i (in spatial
+    // order only without exclusion ranges, and for the self flag only when
+    // sources and targets share one order: slot sl then excludes source sl)
+    const int sl = base + k * 64;
+    const int sc = sl < nt ? sl : nt - 1;
+    const int i = tperm ? tperm[sc] : sc;
+    const long b = (long)i * t_sp;
     const double px = tpos[b], py = tpos[b + t_sc], pz = tpos[b + 2 * t_sc];
-    if (i < nt) {
+    if (sl < nt) {
       bl[0] = fmin(bl[0], px); bh[0] = fmax(bh[0], px);
       bl[1] = fmin(bl[1], py); bh[1] = fmax(bh[1], py);
       bl[2] = fmin(bl[2], pz); bh[2] = fmax(bh[2], pz);
@@ -539,9 +743,9 @@ pairsum_kernel(const T* __restrict__ rec, const T* __restrict__ rec_far, int chu
     }
     lo[k] = 0;
     hi[k] = 0;
-    if ((excl_lo != nullptr || self_excl) && i < nt) {
-      lo[k] = self_excl ? i : excl_lo[i];
-      hi[k] = self_excl ? i + 1 : excl_hi[i];
+    if ((excl_lo != nullptr || self_excl) && sl < nt) {
+      lo[k] = self_excl ? sl : excl_lo[i];
+      hi[k] = self_excl ? sl + 1 : excl_hi[i];
       if (hi[k] > lo[k]) {
         wlo = min(wlo, lo[k]);
         whi = max(whi, hi[k]);
@@ -600,21 +804,21 @@ pairsum_kernel(const T* __restrict__ rec, const T* __restrict__ rec_far, int chu
       }
     } else if (masked) {
       if (CLAMP)
-        tile<T, KIND, K, MODE_CLAMP, true>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax,
-                                           ay, az);
+        tile<T, KIND, K, MODE_CLAMP, true>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2,
+                                           rmax, ax, ay, az);
       else
-        tile<T, KIND, K, MODE_EXACT, true>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax,
-                                           ay, az);
+        tile<T, KIND, K, MODE_EXACT, true>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2,
+                                           rmax, ax, ay, az);
     } else if (CLAMP) {
-      tile<T, KIND, K, MODE_CLAMP, false>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax, ay,
-                                          az);
-    } else if (tile<T, KIND, K, MODE_VOTE, false>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax,
-                                                  ax, ay, az)) {
+      tile<T, KIND, K, MODE_CLAMP, false>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2,
+                                          rmax, ax, ay, az);
+    } else if (tile<T, KIND, K, MODE_VOTE, false>(rec, j0, ns, tx, ty, tz, lo, hi, reg2,
+                                                  eps2, rmax, ax, ay, az)) {
       // some lane met a pair with r < eps: redo the tile with the regularised form
 #pragma unroll
       for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = 0;
-      tile<T, KIND, K, MODE_EXACT, false>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2, rmax, ax, ay,
-                                          az);
+      tile<T, KIND, K, MODE_EXACT, false>(rec, j0, ns, tx, ty, tz, lo, hi, reg2, eps2,
+                                          rmax, ax, ay, az);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -699,8 +903,9 @@ constexpr int kRedTargets = 64;
 template <typename T>
 __global__ void __launch_bounds__(256)
 reduce_kernel(const T* __restrict__ part_hi, const T* __restrict__ part_lo, int P, int nt,
-              double factor, const Geo* __restrict__ geo, int degree, double* __restrict__ out,
-              long o_sp, long o_sc, int accumulate, ShellEpi epi) {
+              const int* __restrict__ tperm, double factor, const Geo* __restrict__ geo,
+              int degree, double* __restrict__ out, long o_sp, long o_sc, int accumulate,
+              ShellEpi epi) {
   if (geo) factor *= degree == 1 ? geo->inv_h : geo->inv_h * geo->inv_h;
   double sigma = 0.0;
   if (epi.sigma_part) {
@@ -713,14 +918,14 @@ reduce_kernel(const T* __restrict__ part_hi, const T* __restrict__ part_lo, int 
   }
   __shared__ double sums[3][3][kRedTargets];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int i = blockIdx.x * kRedTargets + lane;
+  const int sl = blockIdx.x * kRedTargets + lane;  // partial slot
   double a[3] = {0.0, 0.0, 0.0};
-  if (i < nt) {
+  if (sl < nt) {
 #pragma unroll 2
     for (int p = wv; p < P; p += 4) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const size_t idx = ((size_t)p * 3 + c) * nt + i;
+        const size_t idx = ((size_t)p * 3 + c) * nt + sl;
         a[c] += (double)part_hi[idx];
         if (sizeof(T) == 4) a[c] += (double)part_lo[idx];
       }
@@ -731,7 +936,8 @@ reduce_kernel(const T* __restrict__ part_hi, const T* __restrict__ part_lo, int 
     for (int c = 0; c < 3; ++c) sums[wv - 1][c][lane] = a[c];
   }
   __syncthreads();
-  if (wv != 0 || i >= nt) return;
+  if (wv != 0 || sl >= nt) return;
+  const int i = tperm ? tperm[sl] : sl;  // the slot's target
   double u[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -766,6 +972,7 @@ struct Plan {
   int chunk;  // sources per chunk
   int G;      // chunks per block (waves of a block, combined in LDS)
   int Pg;     // partial sums per target: ceil(P / G)
+  int sort;   // sources and targets in spatial order
 };
 
 static void set_groups(Plan& pl) {
@@ -828,6 +1035,14 @@ static Plan make_plan(int kernel, int precision, int64_t ns, int64_t nt) {
   pl.chunk = (int)(tiles_per * kTile);
   pl.P = (int)ceil_div(pl.ns_pad, pl.chunk);
   set_groups(pl);
+  // FP32C Oseen and rotlet calls with at least kSortMin sources and targets
+  // run in spatial order (F3D_PAIRSUM_SORT=0 never, 1 at every size).  Not
+  // the stresslet: its FP32 terms cancel (d.n d.q), and the larger tile sums
+  // of spatially grouped sources raise its error by about 15%, past 1e-6 on
+  // clustered points.
+  const int sort_env = env_int("F3D_PAIRSUM_SORT", -1);
+  pl.sort = precision != F3D_F64 && kernel != F3D_STRESSLET &&
+            (sort_env >= 0 ? sort_env != 0 : (ns >= kSortMin && nt >= kSortMin));
   return pl;
 }
 
@@ -848,11 +1063,15 @@ static size_t part_bytes(int precision, int P, int64_t nt) {
   return align_up((size_t)P * 3 * (size_t)(nt > 0 ? nt : 1) * per, 256);
 }
 
+static size_t sort_bytes(const Plan& pl, int64_t nt) {
+  return pl.sort ? sort_set_bytes(pl.ns_pad) + sort_set_bytes(nt) + sort_count_bytes() : 0;
+}
+
 static size_t plan_ws(int kernel, int precision, const Plan& pl, int64_t nt) {
   return rec_bytes(kernel, precision, pl.ns_pad) + tile_bytes(pl.ns_pad) +
          part_bytes(precision, pl.Pg, nt) +
          align_up(kPackBlocks * sizeof(double), 256) + align_up(sizeof(Geo), 256) +
-         align_up(kBoxBlocks * 6 * sizeof(double), 256);
+         align_up(kBoxBlocks * 6 * sizeof(double), 256) + sort_bytes(pl, nt);
 }
 
 static size_t pairsum_ws(int kernel, int precision, int64_t ns, int64_t nt) {
@@ -904,7 +1123,9 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
   double* sigma_part = reinterpret_cast<double*>(p);
   p += align_up(kPackBlocks * sizeof(double), 256);
   Geo* geo = kQuant ? reinterpret_cast<Geo*>(p) : nullptr;
-  double* box_part = reinterpret_cast<double*>(p + align_up(sizeof(Geo), 256));
+  p += align_up(sizeof(Geo), 256);
+  double* box_part = reinterpret_cast<double*>(p);
+  p += align_up(kBoxBlocks * 6 * sizeof(double), 256);
 
   if (kQuant) {
     // the quantum must resolve eps; an extent below 64 (eps + reg) is raised
@@ -914,13 +1135,34 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
     F3D_LAUNCH(bbox_final_kernel, dim3(1), dim3(kBoxBlocks), 0, st, (const double*)box_part,
                min_extent, geo);
   }
+  // spatial order of sources and targets (one order when they are the same points)
+  const int* sperm = nullptr;
+  const int* tperm = nullptr;
+  // (exclusion ranges keep the given order; the self flag needs one shared
+  // order, in which slot i's own source is source i)
+  const bool same = pos == tpos && s_sp == t_sp && s_sc == t_sc && ns == nt;
+  const bool self_excl_flag = (flags & F3D_EXCLUDE_SELF) != 0;
+  if (kQuant && pl.sort && lo == nullptr && (!self_excl_flag || same)) {
+    const SortBufs bs = sort_set(p, pl.ns_pad);
+    p += sort_set_bytes(pl.ns_pad);
+    const SortBufs bt = sort_set(p, nt);
+    p += sort_set_bytes(nt);
+    int* counts = reinterpret_cast<int*>(p);
+    int rc = spatial_order(ns, pos, s_sp, s_sc, geo, bs, counts, st, &sperm);
+    if (rc != F3D_OK) return rc;
+    tperm = sperm;
+    if (!same) {
+      rc = spatial_order(nt, tpos, t_sp, t_sc, geo, bt, counts, st, &tperm);
+      if (rc != F3D_OK) return rc;
+    }
+  }
   const bool want_sigma = epi_in != nullptr;
   const int tiles = pl.ns_pad / kTile;
   F3D_LAUNCH(tile_box_kernel, dim3((unsigned)tiles), dim3(64), 0, st, (int)ns, tiles, pos,
-             (long)s_sp, (long)s_sc, (const Geo*)geo, tbox);
+             (long)s_sp, (long)s_sc, (const Geo*)geo, sperm, tbox);
   F3D_LAUNCH((pack_kernel<T, KIND>), dim3(kPackBlocks), dim3(256), 0, st, (int)ns, pl.ns_pad,
              pos, nrm, den, w, (long)s_sp, (long)s_sc, (const Geo*)geo,
-             (const TileBox*)tbox, rec, rec_far, want_sigma ? sigma_part : nullptr);
+             (const TileBox*)tbox, sperm, rec, rec_far, want_sigma ? sigma_part : nullptr);
 
   const dim3 grid((unsigned)ceil_div(nt, (int64_t)64 * pl.K), (unsigned)pl.Pg);
   const dim3 block((unsigned)(64 * pl.G));
@@ -934,7 +1176,8 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
   if (timed) F3D_HIP_CHECK(hipEventRecord(g_timer.ev[2 * g_timer.n], st));
 #define F3D_PS(KK, CL)                                                                        \
   F3D_LAUNCH((pairsum_kernel<T, KIND, KK, CL>), grid, block, 0, st, rec, (const T*)rec_far,  \
-             pl.chunk, (int)ns, pl.ns_pad, tpos, (long)t_sp, (long)t_sc, (int)nt, lo, hi,     \
+             tperm, pl.chunk, (int)ns, pl.ns_pad, tpos, (long)t_sp, (long)t_sc,               \
+             (int)nt, lo, hi,                                                                 \
              self_excl, reg, eps, (const Geo*)geo, (const TileBox*)tbox, far_ratio, part_hi,  \
              part_lo)
   if (clamp) {
@@ -960,9 +1203,10 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
     epi = *epi_in;
     epi.sigma_part = sigma_part;
   }
-  F3D_LAUNCH((reduce_kernel<T>), dim3((unsigned)ceil_div(nt, (int64_t)kRedTargets)), dim3(256), 0, st, part_hi,
-             part_lo, pl.Pg, (int)nt, factor, (const Geo*)geo, Degree<KIND>::value, out,
-             (long)o_sp, (long)o_sc, (flags & F3D_ACCUMULATE) ? 1 : 0, epi);
+  F3D_LAUNCH((reduce_kernel<T>), dim3((unsigned)ceil_div(nt, (int64_t)kRedTargets)), dim3(256),
+             0, st, part_hi, part_lo, pl.Pg, (int)nt, tperm, factor, (const Geo*)geo,
+             Degree<KIND>::value, out, (long)o_sp, (long)o_sc, (flags & F3D_ACCUMULATE) ? 1 : 0,
+             epi);
   return F3D_OK;
 }
 
@@ -996,9 +1240,10 @@ static int pairsum_impl(int kernel, int precision, uint32_t flags, int64_t ns, c
   if (ns == 0 && epi == nullptr) {
     if (flags & F3D_ACCUMULATE) return F3D_OK;
     // out = 0 on the caller's layout: reuse the reduce kernel with P = 0.
-    F3D_LAUNCH((reduce_kernel<double>), dim3((unsigned)ceil_div(nt, (int64_t)kRedTargets)), dim3(256), 0, st,
-               (const double*)nullptr, (const double*)nullptr, 0, (int)nt, 0.0,
-               (const Geo*)nullptr, 1, out, (long)o_sp, (long)o_sc, 0, ShellEpi{});
+    F3D_LAUNCH((reduce_kernel<double>), dim3((unsigned)ceil_div(nt, (int64_t)kRedTargets)),
+               dim3(256), 0, st, (const double*)nullptr, (const double*)nullptr, 0, (int)nt,
+               (const int*)nullptr, 0.0, (const Geo*)nullptr, 1, out, (long)o_sp, (long)o_sc, 0,
+               ShellEpi{});
     return F3D_OK;
   }
   F3D_REQUIRE(pos != nullptr && den != nullptr, F3D_EINVAL, "pairsum: NULL source arrays");
