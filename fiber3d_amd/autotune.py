@@ -7,7 +7,9 @@ the launch shapes -- targets per lane K in {1, 2, 4} and the wavefront count
 the source split aims for -- and keeps the fastest.  apply() installs the
 fastest shape of the nearest tuned size (in log2 n_targets + log2 n_sources)
 into every size bucket of the library (f3d_pairsum_set_launch_bucket), and
-the 65536 x 65536 shape as the (kernel, precision) default.
+the 65536 x 65536 shape as the (kernel, precision) default.  The large
+fiber -> fiber Oseen sum (262144 nodes, exclusion ranges) is tuned on its
+own layout, bucket 18/18.
 
 Batched LU (f3d_lu_set_launch): waves per CU of the one-wave factorisation,
 and LDS-staged versus streaming solves, timed on fiber blocks of the shapes
@@ -191,6 +193,46 @@ def tune_pairsum(kernels=("oseen", "stresslet", "rotlet"), precisions=("f32c", "
     return defaults, buckets
 
 
+def tune_pairsum_fibers(precisions=("f32c", "f64"), ks=KS, waves=WAVES, log=None):
+    """The fiber -> fiber Oseen sum of the large workload (4096 fibers x 64
+    nodes in a ball of radius 14, each fiber's own nodes excluded): its
+    spatially compact tiles and exclusion ranges (the given order is kept)
+    make the best shape differ from the unit cube's; tuned at 262144 x
+    262144, bucket 18/18."""
+    import torch
+    from . import configs
+    from .kernels import pairsum
+
+    cfg = configs.config(4)
+    fibers = configs.make_fibers(cfg.fibers, cfg.seed)
+    x = np.concatenate(fibers)
+    lens = np.array([len(f) for f in fibers])
+    lo = np.repeat(np.cumsum(lens) - lens, lens).astype(np.int32)
+    hi = (lo + np.repeat(lens, lens)).astype(np.int32)
+    X = torch.from_numpy(x).cuda()
+    F = torch.from_numpy(np.random.default_rng(1).normal(size=x.shape)).cuda()
+    excl = (torch.from_numpy(lo).cuda(), torch.from_numpy(hi).cuda())
+    out = torch.empty_like(X)
+    n = x.shape[0]
+    res = {}
+    for precision in precisions:
+        best = None
+        for k in ks:
+            for w in waves:
+                set_launch_bucket("oseen", precision, lg(n), lg(n), k, w)
+                t = _time(lambda: pairsum("oseen", X, F, X, excl=excl, precision=precision,
+                                          out=out), min_s=0.05, max_reps=5)
+                rate = n * n / t / 1e9
+                if best is None or rate > best[0]:
+                    best = (rate, k, w)
+        set_launch_bucket("oseen", precision, lg(n), lg(n), best[1], best[2])
+        ent = {"k": best[1], "waves": best[2], "gpairs_s": round(best[0], 1)}
+        res[f"oseen/{precision}"] = {f"{lg(n)}/{lg(n)}": ent}
+        if log:
+            log(f"oseen/{precision} fibers n={n}: {ent}")
+    return res
+
+
 def _lu_workload(seed=0):
     """Fiber blocks of the stepper's shapes: 1000 fibers of 32 nodes
     (n = 128) and 512 of 64 nodes (n = 256), A_sq from the discretisation."""
@@ -261,14 +303,34 @@ def tune_gemm(M=262144, K=256, log=None):
     return res
 
 
-def tune_all(path=CACHE, log=print):
+PARTS = ("pairsum", "fibers", "lu", "gemm")
+
+
+def tune_all(path=CACHE, log=print, parts=PARTS, base=None):
+    """Tune `parts` and save the table; parts not re-tuned are kept from
+    `base` (a table, e.g. the saved one)."""
     import torch
-    defaults, buckets = tune_pairsum(log=log)
-    table = {"pairsum": defaults, "pairsum_buckets": buckets, "lu": tune_lu(log=log),
-             "gemm": tune_gemm(log=log), "device": torch.cuda.get_device_name(0),
-             "problem": "pair sums: unit-cube points, n_targets x n_sources over "
-                        f"{list(GRID_NT)} x {list(GRID_NS)}; LU: 1000 blocks of order 128 "
-                        "and 512 of order 256; GEMM: 262144 x 256 x N"}
+    table = dict(base or {})
+    if "pairsum" in parts:
+        defaults, buckets = tune_pairsum(log=log)
+        table["pairsum"] = defaults
+        old_fib = {k: {s: v for s, v in b.items() if s == "18/18"}
+                   for k, b in table.get("pairsum_buckets", {}).items()}
+        table["pairsum_buckets"] = buckets
+        for k, b in old_fib.items():
+            table["pairsum_buckets"].setdefault(k, {}).update(b)
+    if "fibers" in parts:
+        for k, b in tune_pairsum_fibers(log=log).items():
+            table.setdefault("pairsum_buckets", {}).setdefault(k, {}).update(b)
+    if "lu" in parts:
+        table["lu"] = tune_lu(log=log)
+    if "gemm" in parts:
+        table["gemm"] = tune_gemm(log=log)
+    table["device"] = torch.cuda.get_device_name(0)
+    table["problem"] = ("pair sums: unit-cube points, n_targets x n_sources over "
+                        f"{list(GRID_NT)} x {list(GRID_NS)}, and the 262144-node fiber -> "
+                        "fiber Oseen sum with exclusion ranges (bucket 18/18); LU: 1000 "
+                        "blocks of order 128 and 512 of order 256; GEMM: 262144 x 256 x N")
     validate(table)
     if path:
         with open(path, "w") as fh:
@@ -277,5 +339,14 @@ def tune_all(path=CACHE, log=print):
 
 
 if __name__ == "__main__":
-    import sys
-    tune_all(path=sys.argv[1] if len(sys.argv) > 1 else CACHE)
+    import argparse
+    ap = argparse.ArgumentParser(description="tune launch shapes; writes a tuning table")
+    ap.add_argument("path", nargs="?", default=CACHE)
+    ap.add_argument("--only", nargs="+", choices=PARTS, default=list(PARTS),
+                    help="re-tune these parts, keeping the others from the saved table")
+    a = ap.parse_args()
+    base = None
+    if os.path.exists(CACHE):
+        with open(CACHE) as fh:
+            base = json.load(fh)
+    tune_all(path=a.path, parts=tuple(a.only), base=base)

@@ -264,34 +264,36 @@ radix_count_kernel(int n, int chunk, int shift, const unsigned* __restrict__ key
   counts[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of m = 256 * nb counts in place; each thread owns a
-// contiguous run of m / kSortScan (a multiple of 4) counts
+// exclusive scan of the 256 x kSortMaxBlocks counts in place; each thread
+// owns a contiguous run of kScanPer counts, loaded at once (one latency)
+constexpr int kScanPer = 256 * kSortMaxBlocks / kSortScan;
+static_assert(kScanPer % 4 == 0, "scan runs are whole int4s");
 __global__ void __launch_bounds__(kSortScan)
-radix_scan_kernel(int m, int* __restrict__ counts) {
+radix_scan_kernel(int* __restrict__ counts) {
   __shared__ int part[kSortScan];
-  const int per = m / kSortScan;
-  int4* __restrict__ c4 = reinterpret_cast<int4*>(counts + (long)threadIdx.x * per);
+  int4* __restrict__ c4 = reinterpret_cast<int4*>(counts + (long)threadIdx.x * kScanPer);
+  int4 v[kScanPer / 4];
+#pragma unroll
+  for (int e = 0; e < kScanPer / 4; ++e) v[e] = c4[e];
   int s = 0;
-  for (int e = 0; e < per / 4; ++e) {
-    const int4 v = c4[e];
-    s += v.x + v.y + v.z + v.w;
-  }
+#pragma unroll
+  for (int e = 0; e < kScanPer / 4; ++e) s += v[e].x + v[e].y + v[e].z + v[e].w;
   part[threadIdx.x] = s;
   __syncthreads();
   for (int o = 1; o < kSortScan; o <<= 1) {
-    const int v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+    const int add = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
     __syncthreads();
-    part[threadIdx.x] += v;
+    part[threadIdx.x] += add;
     __syncthreads();
   }
   int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
-  for (int e = 0; e < per / 4; ++e) {
-    const int4 v = c4[e];
+#pragma unroll
+  for (int e = 0; e < kScanPer / 4; ++e) {
     int4 o;
-    o.x = run; run += v.x;
-    o.y = run; run += v.y;
-    o.z = run; run += v.z;
-    o.w = run; run += v.w;
+    o.x = run; run += v[e].x;
+    o.y = run; run += v[e].y;
+    o.z = run; run += v[e].z;
+    o.w = run; run += v[e].w;
     c4[e] = o;
   }
 }
@@ -388,7 +390,7 @@ static int spatial_order(int64_t n, const double* pos, int64_t sp, int64_t sc, c
   for (int shift = 0; shift < 3 * kSortBits; shift += 8) {
     F3D_LAUNCH(radix_count_kernel, dim3(nb), dim3(kSortBlock), 0, st, (int)n, chunk, shift,
                (const unsigned*)b.key[cur], counts);
-    F3D_LAUNCH(radix_scan_kernel, dim3(1), dim3(kSortScan), 0, st, 256 * nb, counts);
+    F3D_LAUNCH(radix_scan_kernel, dim3(1), dim3(kSortScan), 0, st, counts);
     F3D_LAUNCH(radix_scatter_kernel, dim3(nb), dim3(kSortBlock), 0, st, (int)n, chunk, shift,
                (const int*)counts, (const unsigned*)b.key[cur], (const int*)b.val[cur],
                b.key[cur ^ 1], b.val[cur ^ 1]);
