@@ -25,15 +25,17 @@ def _saved():
 
 def test_saved_table_is_valid_and_complete():
     """Every kernel and precision has a default and tuned size buckets over
-    the whole tuning grid; the LU and GEMM sections cover both precisions
-    and both layer widths."""
+    the whole tuning grid, the Oseen sum also the large fiber layout's
+    bucket 18/18; the LU and GEMM sections cover both precisions and both
+    layer widths."""
     t = AT.validate(_saved())
     keys = {f"{k}/{p}" for k in AT.KERNELS for p in AT.PRECS}
     assert set(t["pairsum"]) == keys
     assert set(t["pairsum_buckets"]) == keys
     for key in keys:
         sizes = {(int(a), int(b)) for a, b in (s.split("/") for s in t["pairsum_buckets"][key])}
-        assert sizes == {(AT.lg(nt), AT.lg(ns)) for nt in AT.GRID_NT for ns in AT.GRID_NS}
+        grid = {(AT.lg(nt), AT.lg(ns)) for nt in AT.GRID_NT for ns in AT.GRID_NS}
+        assert sizes == (grid | {(18, 18)} if key.startswith("oseen/") else grid)
         assert t["pairsum"][key]["k"] == t["pairsum_buckets"][key]["16/16"]["k"]
     assert set(t["lu"]) == {"f64", "f32c"}
     assert set(t["gemm"]) == {"128", "256"}
