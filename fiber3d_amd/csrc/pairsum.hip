@@ -221,7 +221,7 @@ __device__ __forceinline__ double wave_min_f64(double v) {
 // buffer it reads was written earlier in the same call (no memset).
 constexpr int kSortBits = 5;          // per axis
 constexpr int kSortBlock = 256;       // threads per sort block
-constexpr int kSortMaxBlocks = 256;   // blocks per pass (a multiple of 16)
+constexpr int kSortMaxBlocks = 64;    // blocks per pass (a multiple of 16)
 constexpr int kSortScan = 1024;       // threads of the scan
 constexpr int64_t kSortMin = 65536;   // sources and targets of a call that sorts
 
@@ -270,7 +270,7 @@ constexpr int kScanPer = 256 * kSortMaxBlocks / kSortScan;
 static_assert(kScanPer % 4 == 0, "scan runs are whole int4s");
 __global__ void __launch_bounds__(kSortScan)
 radix_scan_kernel(int* __restrict__ counts) {
-  __shared__ int part[kSortScan];
+  __shared__ int part[2 * (kSortScan / 64)];
   int4* __restrict__ c4 = reinterpret_cast<int4*>(counts + (long)threadIdx.x * kScanPer);
   int4 v[kScanPer / 4];
 #pragma unroll
@@ -278,15 +278,30 @@ radix_scan_kernel(int* __restrict__ counts) {
   int s = 0;
 #pragma unroll
   for (int e = 0; e < kScanPer / 4; ++e) s += v[e].x + v[e].y + v[e].z + v[e].w;
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = 1; o < kSortScan; o <<= 1) {
-    const int add = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
-    __syncthreads();
-    part[threadIdx.x] += add;
-    __syncthreads();
+  // inclusive scan of the thread sums: within each wave by shuffles, then
+  // over the wave totals
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int up = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += up;
   }
-  int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  if (lane == 63) part[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    // exclusive prefix of the wave totals into part[16 + wave]
+    const int tot = lane < kSortScan / 64 ? part[lane] : 0;
+    int t = tot;
+#pragma unroll
+    for (int o = 1; o < kSortScan / 64; o <<= 1) {
+      const int up = __shfl_up(t, o, 64);
+      if (lane >= o) t += up;
+    }
+    if (lane < kSortScan / 64) part[kSortScan / 64 + lane] = t - tot;
+  }
+  __syncthreads();
+  int run = part[kSortScan / 64 + w] + inc - s;
 #pragma unroll
   for (int e = 0; e < kScanPer / 4; ++e) {
     int4 o;
