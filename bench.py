@@ -394,6 +394,14 @@ def lu_leg(reps=5, cpu=True):
         ts = (time.perf_counter() - t0) / reps
         err = float(torch.linalg.norm(y - xs) / torch.linalg.norm(xs))
         sweeps = int(lu.iters.max()) if prec == "f32c" else 0
+        # the time stepper's solve: no refinement (the preconditioner)
+        yp = lu.solve(b, refine=0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lu.solve(b, out=yp, refine=0)
+        torch.cuda.synchronize()
+        tp = (time.perf_counter() - t0) / reps
         # solve bytes: the factor once per substitution, plus A (FP64) per
         # refinement sweep; sweeps counted at the block maximum
         fb = 8 if prec == "f64" else 4
@@ -408,7 +416,13 @@ def lu_leg(reps=5, cpu=True):
                      "max_refine_sweeps": sweeps,
                      "solve_roofline": {"bound": "hbm", "achieved": round(sbytes / ts / 1e9, 1),
                                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                                        "frac": round(sbytes / ts / 1e9 / PEAK_HBM_GBS, 3)}}
+                                        "frac": round(sbytes / ts / 1e9 / PEAK_HBM_GBS, 3)},
+                     "plain_solve_ms": round(tp * 1e3, 4),
+                     "plain_solve_roofline": {"bound": "hbm",
+                                              "achieved": round(nn * fb / tp / 1e9, 1),
+                                              "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                              "frac": round(nn * fb / tp / 1e9 / PEAK_HBM_GBS,
+                                                            3)}}
     if cpu:
         import scipy.linalg as sla
         blocks = S.bs.unpack(S.A_buf)[:256]
