@@ -270,19 +270,192 @@ gemm_bf16_stream_kernel(int M, int N, int K, const float* __restrict__ A, long s
     }
 }
 
+// ---------------------------------------------------------------------------
+// Resident-weight path for the per-node layers: one persistent 512-thread
+// workgroup per CU holds all of B (K x N, BF16, transposed, 135 KB at
+// N = K = 256) in LDS for the whole launch, so B is read once per CU instead
+// of once per row block, and streams blocks of RM rows of A (32 rows at
+// N = 256, 64 at N = 128).  The FP32 A rows of the next kRD blocks are in
+// flight in registers while a block multiplies: at N = K = 256 that is 64 KB
+// of HBM reads outstanding per CU.  Each of the 8 waves owns one 32 x 32
+// tile of the block (v_mfma_f32_32x32x16_bf16 over K in 16-deep steps, A
+// fragments from the block's LDS rows, B fragments from the resident
+// columns).  K must be a multiple of 16 and at most 256, A row-major with
+// 16-byte aligned rows.
+constexpr int kRKmax = 256;
+constexpr int kRP = kRKmax + 8;  // LDS pitch (bf16): 528 B, 16-byte aligned
+constexpr int kRD = 2;           // A blocks in flight (2, 3, 4, 6 measured alike)
+
+template <int EPI, int NT>  // N = 64 NT (2 or 4)
+__global__ void __launch_bounds__(512)
+gemm_bf16_resident_kernel(int M, int N, int K, const float* __restrict__ A, long sam,
+                          const float* __restrict__ B, long sbk, long sbn,
+                          float* __restrict__ C, long scm, const float* __restrict__ bias,
+                          float* __restrict__ Z, const float* __restrict__ Zin, float alpha) {
+  constexpr int NW = 64 * NT;        // columns
+  constexpr int CT = NW / 32;        // column tiles
+  constexpr int RT = 8 / CT;         // row tiles per block
+  constexpr int RM = 32 * RT;        // rows per block
+  constexpr int PP = RM * kRKmax / 4 / 512;  // float4 pieces per thread (K = 256)
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[NW][kRP];
+  __shared__ __attribute__((aligned(16))) unsigned short As[RM][kRP];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int ct = wave % CT, rt = wave / CT;
+  const int K4 = K / 4;                       // float4 pieces per row
+  const int pieces = RM * K4;                 // per block (<= 512 PP)
+  const int nblocks = (M + RM - 1) / RM;
+
+  auto pack2 = [](float lo, float hi) __attribute__((always_inline)) {
+    return (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
+  };
+  // A pieces of block bl into registers: piece q = tid + 512 j is row q / K4,
+  // columns 4 (q % K4) ..; rows past M and pieces past the block read row 0
+  // (in bounds) and are zeroed when staged.
+  float4 ra[kRD][PP];
+  auto load = [&](int bl, float4 (&r)[PP]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int j = 0; j < PP; ++j) {
+      const int q = tid + 512 * j;
+      const int row = q / K4, c4 = q - row * K4;
+      const long m = (long)bl * RM + row;
+      const bool ok = q < pieces && m < M;
+      r[j] = *reinterpret_cast<const float4*>(A + (ok ? m : 0) * sam + 4 * (ok ? c4 : 0));
+    }
+  };
+  auto stage = [&](int bl, const float4 (&r)[PP]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int j = 0; j < PP; ++j) {
+      const int q = tid + 512 * j;
+      if (q < pieces) {
+        const int row = q / K4, c4 = q - row * K4;
+        const bool ok = (long)bl * RM + row < M;
+        uint2 u;
+        u.x = ok ? pack2(r[j].x, r[j].y) : 0u;
+        u.y = ok ? pack2(r[j].z, r[j].w) : 0u;
+        *reinterpret_cast<uint2*>(&As[row][4 * c4]) = u;
+      }
+    }
+  };
+
+  // the first kRD blocks' A loads, then B into LDS (transposed, BF16)
+  int bl = blockIdx.x;
+#pragma unroll
+  for (int d = 0; d < kRD; ++d) {
+    const int b = bl + d * (int)gridDim.x;
+    load(b < nblocks ? b : nblocks - 1, ra[d]);
+  }
+  for (int idx = tid; idx < NW * (K / 2); idx += 512) {
+    // (k pair, column), k pairs fastest: consecutive lanes store
+    // consecutive words of one LDS row
+    const int kp = idx % (K / 2), n = idx / (K / 2);
+    const int k = 2 * kp;
+    const float b0 = n < N ? B[(long)k * sbk + (long)n * sbn] : 0.0f;
+    const float b1 = n < N ? B[(long)(k + 1) * sbk + (long)n * sbn] : 0.0f;
+    *reinterpret_cast<unsigned*>(&Bs[n][k]) = pack2(b0, b1);
+  }
+
+  const int n = 32 * ct + (lane & 31);
+  const float bn = (bias && n < N) ? bias[n] : 0.0f;
+  // block i of this workgroup uses register set i % kRD (unrolled by kRD)
+  for (; bl < nblocks; bl += kRD * (int)gridDim.x) {
+#pragma unroll
+    for (int d = 0; d < kRD; ++d) {
+      const int b = bl + d * (int)gridDim.x;
+      if (b >= nblocks) break;
+      __syncthreads();  // the previous block's MFMAs are done with As
+      stage(b, ra[d]);
+      __syncthreads();
+      // this register set's next block (an in-bounds reload past the end)
+      const int nb = b + kRD * (int)gridDim.x;
+      load(nb < nblocks ? nb : nblocks - 1, ra[d]);
+      f32x16_t acc;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+      const int kofs = 8 * (lane >> 5);
+#pragma unroll 4
+      for (int k0 = 0; k0 < K; k0 += 16) {
+        const bf16x8_t fa = __builtin_bit_cast(
+            bf16x8_t, *reinterpret_cast<const uint4*>(&As[32 * rt + (lane & 31)][k0 + kofs]));
+        const bf16x8_t fb = __builtin_bit_cast(
+            bf16x8_t, *reinterpret_cast<const uint4*>(&Bs[32 * ct + (lane & 31)][k0 + kofs]));
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc, 0, 0, 0);
+      }
+      // register i of lane l holds row 8(i/4) + 4(l/32) + i%4, column l%32.
+      // All 16 results are formed before any store, in registers of their
+      // own: a store whose data register is overwritten before it has
+      // issued makes the next write wait for it.
+      const long m0 = (long)b * RM + 32 * rt + 4 * (lane >> 5);
+      float v[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        v[i] = alpha * acc[i];
+        if (EPI == EPI_BIAS_RELU || EPI == EPI_BIAS) v[i] += bn;
+      }
+      auto row_of = [](int i) { return 8 * (i >> 2) + (i & 3); };
+      if ((long)(b + 1) * RM <= M) {
+        // a whole block: no row checks
+        const long base = m0 * scm + n;
+        if (EPI == EPI_RELU_GRAD) {
+          float zin[16];
+#pragma unroll
+          for (int i = 0; i < 16; ++i) zin[i] = Zin[base + row_of(i) * scm];
+#pragma unroll
+          for (int i = 0; i < 16; ++i) C[base + row_of(i) * scm] = zin[i] > 0.0f ? v[i] : 0.0f;
+        } else {
+          if (EPI == EPI_BIAS_RELU && Z) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Z[base + row_of(i) * scm] = v[i];
+          }
+#pragma unroll
+          for (int i = 0; i < 16; ++i)
+            C[base + row_of(i) * scm] = EPI == EPI_BIAS_RELU ? (v[i] > 0.0f ? v[i] : 0.0f) : v[i];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const long m = m0 + row_of(i);
+          if (m >= M) continue;
+          const long ci = m * scm + n;
+          if (EPI == EPI_RELU_GRAD) {
+            C[ci] = Zin[ci] > 0.0f ? v[i] : 0.0f;
+          } else {
+            if (EPI == EPI_BIAS_RELU && Z) Z[ci] = v[i];
+            C[ci] = EPI == EPI_BIAS_RELU ? (v[i] > 0.0f ? v[i] : 0.0f) : v[i];
+          }
+        }
+      }
+    }
+  }
+}
+
 }  // namespace f3d
 
 using namespace f3d;
 
-// Whether the streaming kernel serves N = 128 / N = 256 (set by the
-// autotuner through f3d_gemm_set_stream).
-static int g_gemm_stream[2] = {1, 1};
+// Kernel for the per-node layers N = 128 / N = 256 (set by the autotuner
+// through f3d_gemm_set_stream): 0 tiled, 1 streaming, 2 resident-weight.
+static int g_gemm_stream[2] = {2, 2};
 
 extern "C" int f3d_gemm_set_stream(int32_t n, int32_t enable) {
   F3D_REQUIRE(n == 128 || n == 256, F3D_EINVAL, "gemm_set_stream: N must be 128 or 256");
-  F3D_REQUIRE(enable == 0 || enable == 1, F3D_EINVAL, "gemm_set_stream: enable must be 0 or 1");
+  F3D_REQUIRE(enable >= 0 && enable <= 2, F3D_EINVAL,
+              "gemm_set_stream: kernel must be 0 (tiled), 1 (streaming) or 2 (resident)");
   g_gemm_stream[n == 256 ? 1 : 0] = enable;
   return F3D_OK;
+}
+
+static int gemm_num_cus() {
+  static int cached = 0;
+  if (cached == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+    cached = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  }
+  return cached;
 }
 
 extern "C" int f3d_gemm_bf16(int epi, int64_t M, int64_t N, int64_t K, int32_t ksplit,
@@ -304,9 +477,30 @@ extern "C" int f3d_gemm_bf16(int epi, int64_t M, int64_t N, int64_t K, int32_t k
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   // the streaming path: row-major A (unit K stride) and C (unit N stride),
   // N in {128, 256}, K <= 256, no K split
-  const bool stream_ok = epi != EPI_ATOMIC && ksplit == 1 && sak == 1 && scn == 1 &&
-                         ((N == 128 && g_gemm_stream[0]) || (N == 256 && g_gemm_stream[1])) &&
+  const int mode = (N == 128 || N == 256) ? g_gemm_stream[N == 256 ? 1 : 0] : 0;
+  const bool stream_ok = epi != EPI_ATOMIC && ksplit == 1 && sak == 1 && scn == 1 && mode > 0 &&
                          K <= 256 && M >= kSM;
+  // the resident-weight path also needs K a multiple of 16 and 16-byte rows of A
+  const bool resident_ok = stream_ok && mode == 2 && K > 0 && K % 16 == 0 && sam % 4 == 0 &&
+                           (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+  if (resident_ok) {
+    const int RM = N == 256 ? 32 : 64;
+    const int64_t nblocks = ceil_div(M, (int64_t)RM);
+    const dim3 g((unsigned)std::min<int64_t>(nblocks, gemm_num_cus()));
+#define F3D_R(E, T)                                                                          \
+  F3D_LAUNCH((gemm_bf16_resident_kernel<E, T>), g, dim3(512), 0, st, (int)M, (int)N, (int)K, A, \
+             (long)sam, B, (long)sbk, (long)sbn, C, (long)scm, bias, Z, Zin, alpha)
+#define F3D_RE(E) if (N == 256) F3D_R(E, 4); else F3D_R(E, 2)
+    switch (epi) {
+      case EPI_STORE: F3D_RE(EPI_STORE); break;
+      case EPI_BIAS_RELU: F3D_RE(EPI_BIAS_RELU); break;
+      case EPI_RELU_GRAD: F3D_RE(EPI_RELU_GRAD); break;
+      default: F3D_RE(EPI_BIAS); break;
+    }
+#undef F3D_RE
+#undef F3D_R
+    return F3D_OK;
+  }
   if (stream_ok) {
     const dim3 g((unsigned)ceil_div(M, kSM));
 #define F3D_S(E, T)                                                                            F3D_LAUNCH((gemm_bf16_stream_kernel<E, T>), g, dim3(128 * T), 0, st, (int)M, (int)N, (int)K, A,              (long)sam, B, (long)sbk, (long)sbn, C, (long)scm, bias, Z, Zin, alpha)

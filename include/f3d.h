@@ -95,9 +95,11 @@ int f3d_pairsum_plan(int kernel, int precision, int64_t ns, int64_t nt, int32_t*
 int f3d_lu_set_launch(int precision, int32_t factor_waves_per_cu, int32_t solve_staged);
 
 /*
- * Whether f3d_gemm_bf16 uses its streaming kernel for N == n (128 or 256,
- * row-major operands, K <= 256): enable 1 (the default) or 0 (the tiled
- * kernel).  Set by the autotuner; no reference counterpart.
+ * The kernel f3d_gemm_bf16 uses for N == n (128 or 256, row-major operands,
+ * K <= 256): enable 2 (the default) the resident-weight kernel (B held in
+ * LDS by one persistent workgroup per CU; K a multiple of 16, A rows 16-byte
+ * aligned, else the streaming kernel), 1 the streaming kernel, 0 the tiled
+ * kernel.  Set by the autotuner; no reference counterpart.
  */
 int f3d_gemm_set_stream(int32_t n, int32_t enable);
 

@@ -66,3 +66,48 @@ def test_mlp_training_reduces_loss():
     for _ in range(300):
         last = net.train_step(Xd, Td, lr=1e-3)
     assert last < 0.1 * first
+
+
+@pytest.mark.parametrize("N", [128, 256])
+@pytest.mark.parametrize("M,K", [(4099, 256), (1000, 48), (257, 160)])
+@pytest.mark.parametrize("epi", ["store", "bias_relu", "bias", "relu_grad"])
+def test_per_node_layer_kernels_agree(N, M, K, epi):
+    """The per-node layer kernels (resident-weight, streaming, tiled; chosen
+    per N through f3d_gemm_set_stream) on one problem with each epilogue,
+    partial last row blocks included: each meets the FP64 product of the
+    BF16-rounded inputs, and the Z output of the bias + ReLU epilogue holds
+    the pre-activation."""
+    from fiber3d_amd import _lib as L
+    print(f"case layer kernels {M}x{N}x{K} {epi}", flush=True)
+    rng = np.random.default_rng(M * 7 + N + K)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    bias = rng.normal(size=N).astype(np.float32)
+    zin = rng.normal(size=(M, N)).astype(np.float32)
+    pre = _bf16(A) @ _bf16(B)
+    code = {"store": LG.EPI_STORE, "bias_relu": LG.EPI_BIAS_RELU, "bias": LG.EPI_BIAS,
+            "relu_grad": LG.EPI_RELU_GRAD}[epi]
+    if epi == "store":
+        ref = pre
+    elif epi == "bias":
+        ref = pre + bias
+    elif epi == "bias_relu":
+        ref = np.maximum(pre + bias, 0.0)
+    else:
+        ref = np.where(zin > 0, pre, 0.0)
+    lib = L.load()
+    try:
+        for mode in (2, 1, 0):
+            L.check(lib.f3d_gemm_set_stream(N, mode), "f3d_gemm_set_stream")
+            C = torch.full((M, N), 7.0, device="cuda")
+            Z = torch.full((M, N), 7.0, device="cuda") if epi == "bias_relu" else None
+            LG.gemm(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), C, code,
+                    bias=torch.from_numpy(bias).cuda() if epi in ("bias", "bias_relu") else None,
+                    Z=Z, Zin=torch.from_numpy(zin).cuda() if epi == "relu_grad" else None)
+            torch.cuda.synchronize()
+            assert rel2(C.cpu().numpy(), ref) < 1e-6, mode
+            if Z is not None:
+                assert rel2(Z.cpu().numpy(), pre + bias) < 1e-6, mode
+    finally:
+        from fiber3d_amd import autotune
+        autotune.apply()

@@ -77,7 +77,8 @@ def test_library_rejects_bad_shapes():
     assert lib.f3d_lu_set_launch(L.F32C, 9, 1) != 0
     assert lib.f3d_lu_set_launch(L.F32C, 4, 2) != 0
     assert lib.f3d_gemm_set_stream(192, 1) != 0
-    assert lib.f3d_gemm_set_stream(128, 2) != 0
+    assert lib.f3d_gemm_set_stream(128, 3) != 0
+    assert lib.f3d_gemm_set_stream(128, -1) != 0
     bad = _saved()
     bad["pairsum"]["oseen/f32c"]["k"] = 3
     with pytest.raises(ValueError):
