@@ -363,7 +363,7 @@ gemm_bf16_resident_kernel(int M, int N, int K, const float* __restrict__ A, long
 #pragma unroll
     for (int d = 0; d < kRD; ++d) {
       const int b = bl + d * (int)gridDim.x;
-      if (b >= nblocks) break;
+      if (b >= nblocks) continue;  // (uniform; the last round's spare set)
       __syncthreads();  // the previous block's MFMAs are done with As
       stage(b, ra[d]);
       __syncthreads();
@@ -374,6 +374,9 @@ gemm_bf16_resident_kernel(int M, int N, int K, const float* __restrict__ A, long
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
       const int kofs = 8 * (lane >> 5);
+      // (the compiler reports that it cannot honour this unroll hint in
+      // full, but the loop it emits with it measured 4-8% faster than
+      // without it or fully unrolled to K = 256)
 #pragma unroll 4
       for (int k0 = 0; k0 < K; k0 += 16) {
         const bf16x8_t fa = __builtin_bit_cast(

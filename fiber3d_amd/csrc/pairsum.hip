@@ -32,7 +32,7 @@
 //   * Exclusion ranges [lo_i, hi_i) (same fiber / self term) are tested only in
 //     tiles that intersect the wave's union of ranges; all other tiles run a
 //     mask-free path.
-//   * Spatial order: FP32C Oseen and rotlet calls of at least 65536 sources
+//   * Spatial order: Oseen and rotlet calls of at least 65536 sources
 //     and targets, without exclusion ranges, visit both in Morton order
 //     (sorted on the device each call), so that tiles and waves are compact.
 //   * Far tiles: a tile whose centre is more than twice its radius (and more
@@ -211,7 +211,7 @@ __device__ __forceinline__ double wave_min_f64(double v) {
 }
 
 // ---------------------------------------------------------------------------
-// Spatial order (FP32C Oseen and rotlet, large calls): sources and targets
+// Spatial order (Oseen and rotlet, large calls): sources and targets
 // are visited in the Morton order of their quantised positions, 5 bits per
 // axis (32^3 cells, points of one cell in their original order), so that the
 // 64 sources of a tile and the 64 K targets of a wave are spatially compact
@@ -1052,13 +1052,13 @@ static Plan make_plan(int kernel, int precision, int64_t ns, int64_t nt) {
   pl.chunk = (int)(tiles_per * kTile);
   pl.P = (int)ceil_div(pl.ns_pad, pl.chunk);
   set_groups(pl);
-  // FP32C Oseen and rotlet calls with at least kSortMin sources and targets
-  // run in spatial order (F3D_PAIRSUM_SORT=0 never, 1 at every size).  Not
-  // the stresslet: its FP32 terms cancel (d.n d.q), and the larger tile sums
+  // Oseen and rotlet calls with at least kSortMin sources and targets run in
+  // spatial order (F3D_PAIRSUM_SORT=0 never, 1 at every size).  Not the
+  // stresslet: its FP32 terms cancel (d.n d.q), and the larger tile sums
   // of spatially grouped sources raise its error by about 15%, past 1e-6 on
   // clustered points.
   const int sort_env = env_int("F3D_PAIRSUM_SORT", -1);
-  pl.sort = precision != F3D_F64 && kernel != F3D_STRESSLET &&
+  pl.sort = kernel != F3D_STRESSLET &&
             (sort_env >= 0 ? sort_env != 0 : (ns >= kSortMin && nt >= kSortMin));
   return pl;
 }
@@ -1139,37 +1139,41 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
   p += part_bytes(prec, pl.Pg, nt);
   double* sigma_part = reinterpret_cast<double*>(p);
   p += align_up(kPackBlocks * sizeof(double), 256);
-  Geo* geo = kQuant ? reinterpret_cast<Geo*>(p) : nullptr;
+  // the quantisation geometry: FP32C's coordinates, and the Morton codes of
+  // a sorted call in either precision (FP64 kernels take no geometry)
+  Geo* geo_buf = reinterpret_cast<Geo*>(p);
+  Geo* geo = kQuant ? geo_buf : nullptr;
   p += align_up(sizeof(Geo), 256);
   double* box_part = reinterpret_cast<double*>(p);
   p += align_up(kBoxBlocks * 6 * sizeof(double), 256);
+  const bool same = pos == tpos && s_sp == t_sp && s_sc == t_sc && ns == nt;
+  const bool self_excl_flag = (flags & F3D_EXCLUDE_SELF) != 0;
+  const bool sorted = pl.sort && lo == nullptr && (!self_excl_flag || same);
 
-  if (kQuant) {
+  if (kQuant || sorted) {
     // the quantum must resolve eps; an extent below 64 (eps + reg) is raised
     const double min_extent = 64.0 * (eps + reg) + 1e-300;
     F3D_LAUNCH(bbox_partial_kernel, dim3(kBoxBlocks), dim3(256), 0, st, (int)ns, pos, (long)s_sp,
                (long)s_sc, (int)nt, tpos, (long)t_sp, (long)t_sc, box_part);
     F3D_LAUNCH(bbox_final_kernel, dim3(1), dim3(kBoxBlocks), 0, st, (const double*)box_part,
-               min_extent, geo);
+               min_extent, geo_buf);
   }
   // spatial order of sources and targets (one order when they are the same points)
   const int* sperm = nullptr;
   const int* tperm = nullptr;
   // (exclusion ranges keep the given order; the self flag needs one shared
   // order, in which slot i's own source is source i)
-  const bool same = pos == tpos && s_sp == t_sp && s_sc == t_sc && ns == nt;
-  const bool self_excl_flag = (flags & F3D_EXCLUDE_SELF) != 0;
-  if (kQuant && pl.sort && lo == nullptr && (!self_excl_flag || same)) {
+  if (sorted) {
     const SortBufs bs = sort_set(p, pl.ns_pad);
     p += sort_set_bytes(pl.ns_pad);
     const SortBufs bt = sort_set(p, nt);
     p += sort_set_bytes(nt);
     int* counts = reinterpret_cast<int*>(p);
-    int rc = spatial_order(ns, pos, s_sp, s_sc, geo, bs, counts, st, &sperm);
+    int rc = spatial_order(ns, pos, s_sp, s_sc, geo_buf, bs, counts, st, &sperm);
     if (rc != F3D_OK) return rc;
     tperm = sperm;
     if (!same) {
-      rc = spatial_order(nt, tpos, t_sp, t_sc, geo, bt, counts, st, &tperm);
+      rc = spatial_order(nt, tpos, t_sp, t_sc, geo_buf, bt, counts, st, &tperm);
       if (rc != F3D_OK) return rc;
     }
   }

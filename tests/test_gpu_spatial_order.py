@@ -1,15 +1,16 @@
-"""GPU parity of the FP32-compensated pair sums run in spatial order.
+"""GPU parity of the pair sums run in spatial order.
 
-FP32C Oseen and rotlet calls with at least 65536 sources and targets visit
-both in Morton order (a device radix sort) and evaluate far tiles on
-tile-relative FP32 coordinates.  F3D_PAIRSUM_SORT=1 forces the sort at the
-small sizes here, so each case runs both ways and meets the FP64 oracle at
-the FP32C bar (1e-6, relative 2-norm): ragged sizes, distinct sources and
-targets, the self-sum flag (one shared order, slot i excluding source i),
-and clustered points where most tiles are far.  Calls with exclusion ranges
-keep the given order, and the stresslet never sorts (its FP32 terms cancel,
-and spatially grouped tile sums raise its error by about 15%): their cases
-check the far tiles of the given order.
+Oseen and rotlet calls with at least 65536 sources and targets visit both in
+Morton order (a device radix sort); FP32C evaluates far tiles on
+tile-relative FP32 coordinates, FP64 far tiles skip the near-pair vote.
+F3D_PAIRSUM_SORT=1 forces the sort at the small sizes here, so each case runs
+both ways and meets the FP64 oracle at the bars (relative 2-norm: 1e-6
+FP32C, 1e-10 FP64): ragged sizes, distinct sources and targets, the
+self-sum flag (one shared order, slot i excluding source i), and clustered
+points where most tiles are far.  Calls with exclusion ranges keep the given
+order, and the stresslet never sorts (its FP32 terms cancel, and spatially
+grouped tile sums raise its error by about 15%): their cases check the far
+tiles of the given order.
 """
 import numpy as np
 import pytest
@@ -21,7 +22,7 @@ from oracle import kernels as K
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-6
+TOL = {"f64": 1e-10, "f32c": 1e-6}
 
 
 def _dev(a):
@@ -39,10 +40,11 @@ def _clusters(rng, n, centres, size):
     return c + rng.normal(scale=size, size=(n, 3))
 
 
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
 @pytest.mark.parametrize("kernel", ["oseen", "rotlet", "stresslet"])
 @pytest.mark.parametrize("ns,nt", [(1500, 1300), (333, 2049), (65, 1), (1, 7)])
-def test_sorted_sums_match_oracle(order, kernel, ns, nt):
-    print(f"case spatial {order} {kernel} ns={ns} nt={nt}", flush=True)
+def test_sorted_sums_match_oracle(order, kernel, ns, nt, precision):
+    print(f"case spatial {order} {kernel} {precision} ns={ns} nt={nt}", flush=True)
     rng = np.random.default_rng(ns * 31 + nt)
     centres = rng.uniform(-20, 20, size=(12, 3))
     s = _clusters(rng, ns, centres, 0.7)
@@ -56,9 +58,9 @@ def test_sorted_sums_match_oracle(order, kernel, ns, nt):
         ref = K.rotlet_kernel_source_target(s, t, d)
     else:
         ref = K.stresslet_kernel_source_target(s, t, n, d)
-    got = FK.pairsum(kernel, _dev(s), _dev(d), _dev(t), src_nrm=_dev(n), precision="f32c")
+    got = FK.pairsum(kernel, _dev(s), _dev(d), _dev(t), src_nrm=_dev(n), precision=precision)
     torch.cuda.synchronize()
-    assert rel2(got.cpu().numpy(), ref) < TOL
+    assert rel2(got.cpu().numpy(), ref) < TOL[precision]
 
 
 def test_sorted_exclusion_weights_soa_accumulate(order):
@@ -83,11 +85,12 @@ def test_sorted_exclusion_weights_soa_accumulate(order):
     FK.pairsum("oseen", S, F, S, src_w=_dev(w), excl=(torch.from_numpy(lo).cuda(),
                torch.from_numpy(hi).cuda()), precision="f32c", out=out, accumulate=True)
     torch.cuda.synchronize()
-    assert rel2(out.cpu().numpy().ravel() - 1.5, ref) < TOL
+    assert rel2(out.cpu().numpy().ravel() - 1.5, ref) < TOL["f32c"]
 
 
-def test_sorted_self_sum(order):
-    print(f"case spatial self {order}", flush=True)
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
+def test_sorted_self_sum(order, precision):
+    print(f"case spatial self {order} {precision}", flush=True)
     rng = np.random.default_rng(23)
     x = _clusters(rng, 700, rng.uniform(-8, 8, size=(5, 3)), 1.0)
     f = rng.normal(size=(700, 3))
@@ -95,9 +98,9 @@ def test_sorted_self_sum(order):
     ref = K.rotlet_kernel_source_target(x, x, f, reg=0.0, epsilon_distance=0.0,
                                         exclude=(i, i + 1))
     X = _dev(x)
-    got = FK.pairsum("rotlet", X, _dev(f), X, reg=0.0, eps=0.0, precision="f32c",
+    got = FK.pairsum("rotlet", X, _dev(f), X, reg=0.0, eps=0.0, precision=precision,
                      exclude_self=True).cpu().numpy().ravel()
-    assert rel2(got, ref) < TOL
+    assert rel2(got, ref) < TOL[precision]
 
 
 def test_far_clusters_and_default_policy():
@@ -111,13 +114,15 @@ def test_far_clusters_and_default_policy():
     f = rng.normal(size=(3000, 3))
     ref = K.oseen_kernel_source_target(a, b, f)
     got = FK.pairsum("oseen", _dev(a), _dev(f), _dev(b), precision="f32c").cpu().numpy()
-    assert rel2(got, ref) < TOL
+    assert rel2(got, ref) < TOL["f32c"]
     n = 65536
     x = rng.uniform(0, 1, size=(n, 3))
     g = rng.normal(size=(n, 3))
     X, G = _dev(x), _dev(g)
-    sorted_ = FK.pairsum("oseen", X, G, X, precision="f32c", exclude_self=True).cpu().numpy()
     rows = rng.choice(n, size=200, replace=False)
     ref = K.oseen_kernel_source_target(x, x[rows], g, exclude=(rows.astype(np.int32),
                                                                  rows.astype(np.int32) + 1))
-    assert rel2(sorted_[rows], ref) < TOL
+    for precision in ("f32c", "f64"):
+        sorted_ = FK.pairsum("oseen", X, G, X, precision=precision,
+                             exclude_self=True).cpu().numpy()
+        assert rel2(sorted_[rows], ref) < TOL[precision]
