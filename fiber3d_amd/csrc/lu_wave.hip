@@ -161,6 +161,10 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
     // ---- working copy (FP32: power-of-two equilibration) ------------------
     // Column passes over A read kCb columns per round trip (lane rows
     // lane + 64 s), so a lone wave keeps kCb x 4 loads in flight.
+    // (The loads are unconditional, row and column clamped into the block,
+    // and rows and columns past n are zeroed where the values are used: a
+    // load under a lane mask ends in a branch whose join waits for every
+    // load in flight.)
     auto pass = [&](auto&& use) __attribute__((always_inline)) {
       for (int cb = 0; cb < n; cb += kCb) {
         double v[kCb][4];
@@ -168,12 +172,17 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
         for (int u = 0; u < kCb; ++u)
 #pragma unroll
           for (int s = 0; s < 4; ++s) {
-            const int c = cb + u, r = lane + 64 * s;
-            v[u][s] = (c < n && r < n) ? Ab[r + (long)c * n] : 0.0;
+            const int c = min(cb + u, n - 1), r = min(lane + 64 * s, n - 1);
+            v[u][s] = Ab[r + (long)c * n];
           }
 #pragma unroll
-        for (int u = 0; u < kCb; ++u)
-          if (cb + u < n) use(cb + u, v[u]);
+        for (int u = 0; u < kCb; ++u) {
+          if (cb + u < n) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) v[u][s] = lane + 64 * s < n ? v[u][s] : 0.0;
+            use(cb + u, v[u]);
+          }
+        }
       }
     };
     double sr[4];
@@ -230,8 +239,16 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
       for (int s = 0; s < 4; ++s) {
         const int i = lane + 64 * s;
         pos[s] = i < m ? i : 0x7fff;
+        if (npad == n) {
+          // rows past the panel (i >= m) load a clamped row and are never
+          // used (position 0x7fff): no lane mask around the loads
+          const int R = min(c0 + i, n - 1);
 #pragma unroll
-        for (int c = 0; c < kWp; ++c) a[s][c] = (i < m) ? ld_pad(M, n, c0 + i, c0 + c) : (T)0;
+          for (int c = 0; c < kWp; ++c) a[s][c] = M[R + (long)(c0 + c) * n];
+        } else {
+#pragma unroll
+          for (int c = 0; c < kWp; ++c) a[s][c] = (i < m) ? ld_pad(M, n, c0 + i, c0 + c) : (T)0;
+        }
       }
       int pv[kWp];
       sfor<0, kWp>([&](auto J) __attribute__((always_inline)) {
@@ -362,15 +379,24 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
         if ((left || right) && col < n) {
         T* __restrict__ cp = M + (long)col * n;
         T xt[kWp], xl[kWp];
+        if (npad == n) {
+          // every row is real: unconditional loads (the values of pivots
+          // inside the top 16 are not stored)
 #pragma unroll
-        for (int r = 0; r < kWp; ++r) {
-          const int R = c0 + src_top[r];
-          xt[r] = R < n ? cp[R] : (T)0;
-        }
+          for (int r = 0; r < kWp; ++r) xt[r] = cp[c0 + src_top[r]];
 #pragma unroll
-        for (int j = 0; j < kWp; ++j) {
-          const int R = c0 + src_low[j];
-          xl[j] = (pv[j] >= kWp && R < n) ? cp[R] : (T)0;
+          for (int j = 0; j < kWp; ++j) xl[j] = cp[c0 + src_low[j]];
+        } else {
+#pragma unroll
+          for (int r = 0; r < kWp; ++r) {
+            const int R = c0 + src_top[r];
+            xt[r] = R < n ? cp[R] : (T)0;
+          }
+#pragma unroll
+          for (int j = 0; j < kWp; ++j) {
+            const int R = c0 + src_low[j];
+            xl[j] = (pv[j] >= kWp && R < n) ? cp[R] : (T)0;
+          }
         }
         if (right) {
           // forward substitution with the unit-lower L11 (LDS, broadcast)
