@@ -336,10 +336,21 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
         if (i < m) {
           const int P = pos[s];
           sidx[P] = i;
+          if (npad == n) {
+            // every row and column is real: the stores need no guards (a
+            // guard per store makes each wait for the one before)
+            T* __restrict__ dst = M + c0 + P + (long)c0 * n;
 #pragma unroll
-          for (int c = 0; c < kWp; ++c) {
-            sL[c * kWs + P] = a[s][c];
-            if (c0 + P < n && c0 + c < n) M[c0 + P + (long)(c0 + c) * n] = a[s][c];
+            for (int c = 0; c < kWp; ++c) {
+              sL[c * kWs + P] = a[s][c];
+              dst[(long)c * n] = a[s][c];
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < kWp; ++c) {
+              sL[c * kWs + P] = a[s][c];
+              if (c0 + P < n && c0 + c < n) M[c0 + P + (long)(c0 + c) * n] = a[s][c];
+            }
           }
         }
       }
@@ -408,9 +419,14 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
             });
           });
         }
+        if (npad == n) {
 #pragma unroll
-        for (int r = 0; r < kWp; ++r)
-          if (c0 + r < n) cp[c0 + r] = xt[r];
+          for (int r = 0; r < kWp; ++r) cp[c0 + r] = xt[r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < kWp; ++r)
+            if (c0 + r < n) cp[c0 + r] = xt[r];
+        }
 #pragma unroll
         for (int j = 0; j < kWp; ++j)
           if (pv[j] >= kWp && c0 + pv[j] < n) cp[c0 + pv[j]] = xl[j];
