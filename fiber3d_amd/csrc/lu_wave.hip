@@ -196,24 +196,23 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
       });
 #pragma unroll
       for (int s = 0; s < 4; ++s) sr[s] = pow2_scale_w(mr[s]);
-      // column maxima of the row-scaled matrix -> scl_c (LDS)
+      // column maxima of the row-scaled matrix -> scl_c (LDS), and in the
+      // same pass the scaled column (its scale is known once its maximum is)
       pass([&](int c, const double (&v)[4]) {
         double m = 0.0;
 #pragma unroll
         for (int s = 0; s < 4; ++s) m = fmax(m, fabs(v[s]) * sr[s]);
         m = row_max(m);
         m = fmax(fmax(rl(m, 0), rl(m, 16)), fmax(rl(m, 32), rl(m, 48)));
-        if (lane == 0) scl_c[c] = pow2_scale_w(m);
-      });
-      __syncthreads();
-      pass([&](int c, const double (&v)[4]) {
-        const double scol = scl_c[c];
+        const double scol = pow2_scale_w(m);
+        if (lane == 0) scl_c[c] = scol;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const int r = lane + 64 * s;
           if (r < n) M[r + (long)c * n] = (T)(v[s] * sr[s] * scol);
         }
       });
+      __syncthreads();
     } else {
       pass([&](int c, const double (&v)[4]) {
 #pragma unroll
