@@ -421,14 +421,21 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
         if (npad == n) {
 #pragma unroll
           for (int r = 0; r < kWp; ++r) cp[c0 + r] = xt[r];
+          // the rows swapped in from below the top 16; a pivot inside the
+          // top 16 stores top row j's own value again (no branch per store)
+#pragma unroll
+          for (int j = 0; j < kWp; ++j) {
+            const bool low = pv[j] >= kWp;
+            cp[c0 + (low ? pv[j] : j)] = low ? xl[j] : xt[j];
+          }
         } else {
 #pragma unroll
           for (int r = 0; r < kWp; ++r)
             if (c0 + r < n) cp[c0 + r] = xt[r];
-        }
 #pragma unroll
-        for (int j = 0; j < kWp; ++j)
-          if (pv[j] >= kWp && c0 + pv[j] < n) cp[c0 + pv[j]] = xl[j];
+          for (int j = 0; j < kWp; ++j)
+            if (pv[j] >= kWp && c0 + pv[j] < n) cp[c0 + pv[j]] = xl[j];
+        }
         }
       }
       __syncthreads();  // U12 and the swapped rows visible to the tile loads
