@@ -26,7 +26,12 @@
 //      group of 16 lanes loads and stores 16 consecutive rows of one column:
 //      A' = -U12^T (global), B' = L21^T (LDS), C/D from/to the factor buffer,
 //      (FP64 output rows interleave, so those groups hold columns 4 e + g),
-//      with the loads of the next tiles in flight while a tile multiplies.
+//      one tile column at a time with all its loads in flight before its
+//      first MFMA.
+// Loads and stores are kept out of lane-masked branches where the order is a
+// multiple of 16 (clamped indices, values past the block unused): a memory
+// operation under a mask ends in a branch whose join waits for every memory
+// operation in flight.
 // FP32 blocks are first equilibrated by powers of two (exact) per row and
 // column; the solves refine in FP64 against the unscaled FP64 matrix.
 // Orders that are not a multiple of 16 are padded with an identity block that
@@ -137,7 +142,7 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
                                              int* __restrict__ info) {
   typedef typename Mf16<T>::v4 v4;
   constexpr bool kEq = sizeof(T) == 4;  // FP32: equilibrate
-  // trailing-update tiles with loads in flight (FP32 keeps two waves per SIMD)
+  // padded orders: trailing-update tiles with loads in flight
   constexpr int kRing = sizeof(T) == 8 ? 16 : 8;
   __shared__ T sL[kWp * kWs];           // panel, column c at c * kWs + local row
   __shared__ int sperm[kWmax];
@@ -468,6 +473,37 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
               ++ct;
             }
           };
+          if constexpr (FULL) {
+            // One tile column at a time: the column's U12 strip, then all
+            // its tiles (rows past the last tile reload the last one, in
+            // bounds and unused), issue before its first MFMA with no branch
+            // between, so a column waits for memory once.
+            constexpr int RMAX = kWmax / kWp - 1;
+            for (int ct = 0; ct < nr; ++ct) {
+              const int C0 = R00 + kWp * ct;
+              T ua[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) ua[q] = -M[c0 + g + 4 * q + (C0 + e16) * n];
+              v4 tile[RMAX];
+#pragma unroll
+              for (int rt = 0; rt < RMAX; ++rt) tile[rt] = load(rt < nr ? rt : nr - 1, ct);
+#pragma unroll
+              for (int rt = 0; rt < RMAX; ++rt) {
+                if (rt < nr) {
+                  v4 acc = tile[rt];
+                  const int R0 = R00 + kWp * rt;
+                  const T* __restrict__ lb = sL + (R0 - c0) + e16;
+#pragma unroll
+                  for (int q = 0; q < 4; ++q)
+                    acc = Mf16<T>::mma(ua[q], lb[(g + 4 * q) * kWs], acc);
+#pragma unroll
+                  for (int e = 0; e < 4; ++e) M[R0 + C0 * n + lc[e]] = acc[e];
+                }
+              }
+            }
+            return;
+          }
+          // (orders not a multiple of 16: tiles with loads in flight in a ring)
           v4 ring[kRing];
           int l_rt = 0, l_ct = 0;  // next tile to load
 #pragma unroll
