@@ -478,26 +478,50 @@ __device__ __forceinline__ void lu_wave_body(int nb, const int* __restrict__ n_a
             // its tiles (rows past the last tile reload the last one, in
             // bounds and unused), issue before its first MFMA with no branch
             // between, so a column waits for memory once.
-            constexpr int RMAX = kWmax / kWp - 1;
+            // tiles loaded together: all 16 >= the 15 trailing tiles of a
+            // column in FP64 (accumulators in AGPRs), 4 in FP32 (two waves
+            // per SIMD share the register file)
+            constexpr int TH = sizeof(T) == 8 ? kWmax / kWp : kWmax / kWp / 4;
             for (int ct = 0; ct < nr; ++ct) {
               const int C0 = R00 + kWp * ct;
               T ua[4];
 #pragma unroll
               for (int q = 0; q < 4; ++q) ua[q] = -M[c0 + g + 4 * q + (C0 + e16) * n];
-              v4 tile[RMAX];
 #pragma unroll
-              for (int rt = 0; rt < RMAX; ++rt) tile[rt] = load(rt < nr ? rt : nr - 1, ct);
+              for (int h = 0; h < kWmax / kWp; h += TH) {
+                if (h < nr) {
+                  v4 tile[TH];
 #pragma unroll
-              for (int rt = 0; rt < RMAX; ++rt) {
-                if (rt < nr) {
-                  v4 acc = tile[rt];
-                  const int R0 = R00 + kWp * rt;
-                  const T* __restrict__ lb = sL + (R0 - c0) + e16;
+                  for (int u = 0; u < TH; ++u)
+                    tile[u] = load(h + u < nr ? h + u : nr - 1, ct);
+                  // four tiles at a time, their MFMA chains interleaved; a
+                  // tile past the last recomputes the last one from the same
+                  // loaded values and stores the same result again
 #pragma unroll
-                  for (int q = 0; q < 4; ++q)
-                    acc = Mf16<T>::mma(ua[q], lb[(g + 4 * q) * kWs], acc);
+                  for (int r4 = 0; r4 < TH; r4 += 4) {
+                    if (h + r4 < nr) {
+                      v4 acc[4];
+                      const T* __restrict__ lb[4];
+                      int rts[4];
 #pragma unroll
-                  for (int e = 0; e < 4; ++e) M[R0 + C0 * n + lc[e]] = acc[e];
+                      for (int u = 0; u < 4; ++u) {
+                        acc[u] = tile[r4 + u];
+                        rts[u] = h + r4 + u < nr ? h + r4 + u : nr - 1;
+                        lb[u] = sL + (R00 + kWp * rts[u] - c0) + e16;
+                      }
+#pragma unroll
+                      for (int q = 0; q < 4; ++q)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                          acc[u] = Mf16<T>::mma(ua[q], lb[u][(g + 4 * q) * kWs], acc[u]);
+#pragma unroll
+                      for (int u = 0; u < 4; ++u) {
+                        const int R0 = R00 + kWp * rts[u];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) M[R0 + C0 * n + lc[e]] = acc[u][e];
+                      }
+                    }
+                  }
                 }
               }
             }
