@@ -613,16 +613,17 @@ template <typename T, int KIND, int U, class Body>
 __device__ __forceinline__ void for_groups(const T* __restrict__ rec, int j0, Body&& body) {
   constexpr int R = RecLen<KIND>::value, NV = R;
   if constexpr (Prefetch<T, KIND>::value) {
+    // two register sets that trade roles (no copies): the next group loads
+    // into one while the other computes (the last group reloads itself: no
+    // read past the tile)
+    static_assert(kTile % (2 * U) == 0, "groups pair up within a tile");
     T s[U][NV], nx[U][NV];
     load_recs<T, R, NV, U>(rec, j0, s);
-    for (int jj = 0; jj < kTile; jj += U) {
-      // (the last group reloads itself: no read past the tile)
-      load_recs<T, R, NV, U>(rec, j0 + (jj + U < kTile ? jj + U : jj), nx);
+    for (int jj = 0; jj < kTile; jj += 2 * U) {
+      load_recs<T, R, NV, U>(rec, j0 + jj + U, nx);
       body(s, jj);
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int c = 0; c < NV; ++c) s[u][c] = nx[u][c];
+      load_recs<T, R, NV, U>(rec, j0 + (jj + 2 * U < kTile ? jj + 2 * U : jj + U), s);
+      body(nx, jj + U);
     }
   } else {
     for (int jj = 0; jj < kTile; jj += U) {
