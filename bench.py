@@ -48,13 +48,13 @@ PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 # HBM bytes per launch of the FP32-compensated Oseen pair kernel at N = M = 65536:
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --steps 2
-# --warmup 1 --no-cpu --no-lu --no-small --no-series --system -1` (28301.8 and
+# --warmup 1 --no-cpu --no-lu --no-small --no-series --system -1` (28313.3 and
 # 12288 KiB per dispatch), scaled by what the same counters read for
 # `bench.py --copy-probe 512` (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  The writes
 # are the partial sums (32 source chunks, 4 combined per workgroup) that the
 # reduce kernel reads back; the fetches are mostly source records re-read by
 # the 1024 target waves.
-OSEEN_F32C_HBM_BYTES = 70545024
+OSEEN_F32C_HBM_BYTES = 70568576
 
 
 def host_cores():
