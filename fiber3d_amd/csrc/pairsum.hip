@@ -55,15 +55,24 @@ constexpr int kBlock = 256;      // threads per block (4 waves)
 constexpr int kPackBlocks = 256; // grid of the pack kernel (sigma partials)
 constexpr double kQuantLevels = 2147483648.0 - 65536.0;  // 2^31 minus margin
 
+// FP64 Oseen pairs use w = 2/r, one Newton step on v_rsq_f64 without its
+// factor 1/2, and dot d with f/4 (a second copy of the force in the record):
+// w (f + (w^2 (d.f/4)) d) = 2 (f/r + (d.f) d/r^3) in 20 FP64 operations per
+// pair instead of 22.  The sums come out doubled; the reduction halves them.
+template <typename T, int KIND>
+struct TwoOverR {
+  static constexpr bool value = std::is_same<T, double>::value && KIND == F3D_OSEEN;
+};
+
 // Record words per source, unpadded so that a group of records loads with a
 // few wide scalar loads: 6 for Oseen and rotlet (x, y, z, f or tau), 9 for
-// the stresslet (x, y, z, n, q).  FP32C records hold the quantised position
-// as int32 bits in words 0..2; their far records (same layout) hold the
-// position relative to the source tile's quantised centre, in units of the
-// quantum, as FP32.
-template <int KIND>
+// the stresslet (x, y, z, n, q) and for FP64 Oseen (x, y, z, f, f/4).  FP32C
+// records hold the quantised position as int32 bits in words 0..2; their far
+// records (same layout) hold the position relative to the source tile's
+// quantised centre, in units of the quantum, as FP32.
+template <typename T, int KIND>
 struct RecLen {
-  static constexpr int value = (KIND == F3D_STRESSLET) ? 9 : 6;
+  static constexpr int value = (KIND == F3D_STRESSLET || TwoOverR<T, KIND>::value) ? 9 : 6;
 };
 
 // Tile geometry for the near/far split: centre and radius of the sources of
@@ -97,6 +106,12 @@ __device__ __forceinline__ double rsqrt_t(double x) {
   double h = x * y;
   double e = fma(-h, y, 1.0);
   return fma(0.5 * y, e, y);
+}
+// 2/sqrt(x) to the same accuracy: y (3 - x y^2) = 2 y (1.5 - 0.5 x y^2), three
+// operations after v_rsq_f64 where the step above takes four.
+__device__ __forceinline__ double rsqrt2_t(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return y * fma(-x, y * y, 3.0);
 }
 
 // Quantisation geometry of one FP32C call, computed on the device:
@@ -468,7 +483,7 @@ pack_kernel(int ns, int ns_pad, const double* __restrict__ pos, const double* __
             const Geo* __restrict__ geo, const TileBox* __restrict__ tbox,
             const int* __restrict__ perm, T* __restrict__ rec, T* __restrict__ rec_far,
             double* __restrict__ sigma_part) {
-  constexpr int R = RecLen<KIND>::value;
+  constexpr int R = RecLen<T, KIND>::value;
   constexpr bool kQuant = std::is_same<T, float>::value;
   double sig = 0.0;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ns_pad; j += gridDim.x * blockDim.x) {
@@ -493,6 +508,11 @@ pack_kernel(int ns, int ns_pad, const double* __restrict__ pos, const double* __
         r[3] = (T)(den[b] * wj);
         r[4] = (T)(den[b + sc] * wj);
         r[5] = (T)(den[b + 2 * sc] * wj);
+        if constexpr (TwoOverR<T, KIND>::value) {
+          r[6] = 0.25 * r[3];
+          r[7] = 0.25 * r[4];
+          r[8] = 0.25 * r[5];
+        }
       }
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -532,17 +552,29 @@ using TCoord = typename std::conditional<std::is_same<T, float>::value, int, dou
 // near-pair handling of one pair
 enum { MODE_EXACT = 0, MODE_CLAMP = 1, MODE_VOTE = 2 };
 
-// The kernel's contribution for d = (dx, dy, dz), ri = 1/|d| and the
-// source's density words dv (Oseen f; rotlet tau; stresslet n, q).
+// a b + c for a wave-uniform c.  FP64: one VOP3 v_fma_f64 that reads c from
+// its SGPRs; left to itself the compiler turns an FMA whose addend has no
+// other use into v_fmac_f64 after two v_mov copies of c.
+__device__ __forceinline__ float fma_s(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_s(double a, double b, double c) {
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+  return r;
+}
+
+// The kernel's contribution for d = (dx, dy, dz), ri = 1/|d| (FP64 Oseen:
+// 2/|d|) and the source's density words dv (Oseen f, FP64 also f/4; rotlet
+// tau; stresslet n, q).
 template <typename T, int KIND>
 __device__ __forceinline__ void accum(const T* dv, T dx, T dy, T dz, T ri, T& ax, T& ay,
                                       T& az) {
   if (KIND == F3D_OSEEN) {
-    const T fd = dx * dv[0] + dy * dv[1] + dz * dv[2];
+    constexpr int q = TwoOverR<T, KIND>::value ? 3 : 0;
+    const T fd = dx * dv[q] + dy * dv[q + 1] + dz * dv[q + 2];
     const T s = fd * (ri * ri);
-    ax += ri * (dv[0] + s * dx);
-    ay += ri * (dv[1] + s * dy);
-    az += ri * (dv[2] + s * dz);
+    ax += ri * fma_s(s, dx, dv[0]);
+    ay += ri * fma_s(s, dy, dv[1]);
+    az += ri * fma_s(s, dz, dv[2]);
   } else if (KIND == F3D_ROTLET) {
     const T ri3 = ri * ri * ri;
     ax += ri3 * (dv[1] * dz - dv[2] * dy);
@@ -579,7 +611,9 @@ __device__ __forceinline__ void pair(const T* r, TCoord<T> tx, TCoord<T> ty, TCo
   if (MODE == MODE_EXACT) r2 = (r2 < eps2) ? r2 + reg2 : r2;
   // the near-pair vote lives in SGPRs: one v_cmp into a lane mask, OR-ed on the SALU
   if (MODE == MODE_VOTE) vote |= __ballot(r2 < eps2);
-  T ri = rsqrt_t(r2);
+  T ri;
+  if constexpr (TwoOverR<T, KIND>::value) ri = rsqrt2_t(r2);
+  else ri = rsqrt_t(r2);
   if (MODE == MODE_CLAMP) ri = ri < rmax ? ri : rmax;
   if (MASK) ri = excl ? (T)0 : ri;
   accum<T, KIND>(r + 3, dx, dy, dz, ri, ax, ay, az);
@@ -611,7 +645,7 @@ struct Prefetch {
 // body(s, jj) for each group of U records of the 64-source tile at j0.
 template <typename T, int KIND, int U, class Body>
 __device__ __forceinline__ void for_groups(const T* __restrict__ rec, int j0, Body&& body) {
-  constexpr int R = RecLen<KIND>::value, NV = R;
+  constexpr int R = RecLen<T, KIND>::value, NV = R;
   if constexpr (Prefetch<T, KIND>::value) {
     // two register sets that trade roles (no copies): the next group loads
     // into one while the other computes (the last group reloads itself: no
@@ -645,7 +679,7 @@ __device__ __forceinline__ bool tile(const T* __restrict__ rec, int j0, int ns,
                                      const TCoord<T> (&ty)[K], const TCoord<T> (&tz)[K],
                                      const int (&lo)[K], const int (&hi)[K], T reg2, T eps2,
                                      T rmax, T (&ax)[K], T (&ay)[K], T (&az)[K]) {
-  constexpr int NV = RecLen<KIND>::value;
+  constexpr int NV = RecLen<T, KIND>::value;
   unsigned long long vote = 0;
   for_groups<T, KIND, kGroup>(rec, j0, [&](const T (&s)[kGroup][NV], int jj) {
 #pragma unroll
@@ -671,14 +705,17 @@ template <typename T, int KIND, int K>
 __device__ __forceinline__ void tile_far(const T* __restrict__ rec, int j0, const T (&lx)[K],
                                          const T (&ly)[K], const T (&lz)[K], T (&ax)[K],
                                          T (&ay)[K], T (&az)[K]) {
-  constexpr int NV = RecLen<KIND>::value;
+  constexpr int NV = RecLen<T, KIND>::value;
   for_groups<T, KIND, kGroup>(rec, j0, [&](const T (&s)[kGroup][NV], int) {
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const T dx = lx[k] - s[u][0], dy = ly[k] - s[u][1], dz = lz[k] - s[u][2];
-        const T ri = rsqrt_t(dx * dx + dy * dy + dz * dz);
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        T ri;
+        if constexpr (TwoOverR<T, KIND>::value) ri = rsqrt2_t(r2);
+        else ri = rsqrt_t(r2);
         accum<T, KIND>(s[u] + 3, dx, dy, dz, ri, ax[k], ay[k], az[k]);
       }
     }
@@ -722,7 +759,8 @@ pairsum_kernel(const T* __restrict__ rec, const T* __restrict__ rec_far,
   const double hs = kQuant ? geo->h : 1.0;
   const T reg2 = (T)((reg / hs) * (reg / hs));
   const T eps2 = (T)((eps / hs) * (eps / hs));
-  const T rmax = (T)(hs / eps);
+  // (FP64 Oseen clamps w = 2/r)
+  const T rmax = (T)((TwoOverR<T, KIND>::value ? 2.0 : 1.0) * hs / eps);
   // The G = blockDim.x / 64 waves of a block share their 64 K targets and
   // take consecutive source chunks; their sums are combined in LDS, so the
   // block writes one partial per target (G times fewer partial bytes).
@@ -1066,7 +1104,9 @@ static Plan make_plan(int kernel, int precision, int64_t ns, int64_t nt) {
 
 // source records; FP32C adds the far records (same size)
 static size_t rec_bytes(int kernel, int precision, int ns_pad) {
-  const size_t r = kernel == F3D_STRESSLET ? RecLen<F3D_STRESSLET>::value : RecLen<F3D_OSEEN>::value;
+  const size_t r = kernel == F3D_STRESSLET ? RecLen<double, F3D_STRESSLET>::value
+                   : kernel == F3D_OSEEN && precision == F3D_F64 ? RecLen<double, F3D_OSEEN>::value
+                                                                 : RecLen<float, F3D_OSEEN>::value;
   const size_t t = precision == F3D_F64 ? 8 : 4;
   const size_t one = align_up((size_t)ns_pad * r * t, 256);
   return precision == F3D_F64 ? one : 2 * one;
@@ -1220,6 +1260,7 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
   double factor;
   if (KIND == F3D_STRESSLET) factor = -3.0 / (4.0 * kPi * eta);
   else factor = 1.0 / (8.0 * kPi * eta);
+  if (TwoOverR<T, KIND>::value) factor *= 0.5;
   ShellEpi epi{};
   if (epi_in) {
     epi = *epi_in;
