@@ -55,13 +55,17 @@ constexpr int kBlock = 256;      // threads per block (4 waves)
 constexpr int kPackBlocks = 256; // grid of the pack kernel (sigma partials)
 constexpr double kQuantLevels = 2147483648.0 - 65536.0;  // 2^31 minus margin
 
-// FP64 Oseen pairs use w = 2/r, one Newton step on v_rsq_f64 without its
-// factor 1/2, and dot d with f/4 (a second copy of the force in the record):
-// w (f + (w^2 (d.f/4)) d) = 2 (f/r + (d.f) d/r^3) in 20 FP64 operations per
-// pair instead of 22.  The sums come out doubled; the reduction halves them.
+// FP64 pairs use w = 2/r, one Newton step on v_rsq_f64 without its factor
+// 1/2 (one FP64 operation fewer per pair).  The kernels are homogeneous in
+// 1/r, so the sums come out scaled by a power of 2 that the reduction takes
+// back: rotlet w^3 = 8/r^3, stresslet w^5 = 32/r^5.  Oseen dots d with f/4
+// (a second copy of the force in the record) so that its two terms keep
+// their ratio: w (f + w^2 (d.f/4) d) = 2 (f/r + (d.f) d/r^3).
 template <typename T, int KIND>
 struct TwoOverR {
-  static constexpr bool value = std::is_same<T, double>::value && KIND == F3D_OSEEN;
+  static constexpr bool value = std::is_same<T, double>::value;
+  static constexpr double scale =
+      !value ? 1.0 : KIND == F3D_OSEEN ? 0.5 : KIND == F3D_ROTLET ? 0.125 : 1.0 / 32.0;
 };
 
 // Record words per source, unpadded so that a group of records loads with a
@@ -72,7 +76,8 @@ struct TwoOverR {
 // quantised centre, in units of the quantum, as FP32.
 template <typename T, int KIND>
 struct RecLen {
-  static constexpr int value = (KIND == F3D_STRESSLET || TwoOverR<T, KIND>::value) ? 9 : 6;
+  static constexpr int value =
+      (KIND == F3D_STRESSLET || (TwoOverR<T, KIND>::value && KIND == F3D_OSEEN)) ? 9 : 6;
 };
 
 // Tile geometry for the near/far split: centre and radius of the sources of
@@ -508,7 +513,7 @@ pack_kernel(int ns, int ns_pad, const double* __restrict__ pos, const double* __
         r[3] = (T)(den[b] * wj);
         r[4] = (T)(den[b + sc] * wj);
         r[5] = (T)(den[b + 2 * sc] * wj);
-        if constexpr (TwoOverR<T, KIND>::value) {
+        if constexpr (TwoOverR<T, KIND>::value && KIND == F3D_OSEEN) {
           r[6] = 0.25 * r[3];
           r[7] = 0.25 * r[4];
           r[8] = 0.25 * r[5];
@@ -562,9 +567,9 @@ __device__ __forceinline__ double fma_s(double a, double b, double c) {
   return r;
 }
 
-// The kernel's contribution for d = (dx, dy, dz), ri = 1/|d| (FP64 Oseen:
-// 2/|d|) and the source's density words dv (Oseen f, FP64 also f/4; rotlet
-// tau; stresslet n, q).
+// The kernel's contribution for d = (dx, dy, dz), ri = 1/|d| (FP64: 2/|d|)
+// and the source's density words dv (Oseen f, FP64 also f/4; rotlet tau;
+// stresslet n, q).
 template <typename T, int KIND>
 __device__ __forceinline__ void accum(const T* dv, T dx, T dy, T dz, T ri, T& ax, T& ay,
                                       T& az) {
@@ -759,7 +764,7 @@ pairsum_kernel(const T* __restrict__ rec, const T* __restrict__ rec_far,
   const double hs = kQuant ? geo->h : 1.0;
   const T reg2 = (T)((reg / hs) * (reg / hs));
   const T eps2 = (T)((eps / hs) * (eps / hs));
-  // (FP64 Oseen clamps w = 2/r)
+  // (FP64 clamps w = 2/r)
   const T rmax = (T)((TwoOverR<T, KIND>::value ? 2.0 : 1.0) * hs / eps);
   // The G = blockDim.x / 64 waves of a block share their 64 K targets and
   // take consecutive source chunks; their sums are combined in LDS, so the
@@ -1260,7 +1265,7 @@ static int launch_pairsum(const Plan& pl, uint32_t flags, int64_t ns, const doub
   double factor;
   if (KIND == F3D_STRESSLET) factor = -3.0 / (4.0 * kPi * eta);
   else factor = 1.0 / (8.0 * kPi * eta);
-  if (TwoOverR<T, KIND>::value) factor *= 0.5;
+  factor *= TwoOverR<T, KIND>::scale;
   ShellEpi epi{};
   if (epi_in) {
     epi = *epi_in;
