@@ -22,6 +22,8 @@
 // -> every thread swaps, forms its multipliers and updates its tile.
 #include "common.h"
 
+#include <type_traits>
+
 #include <stdlib.h>
 #include <string.h>
 
@@ -555,11 +557,15 @@ __device__ __forceinline__ void permute_rows(double (&w)[kSlots], const int (&sr
 
 // Column streaming for the one-wave solves: column k of a column-major n x n
 // block is consumed in order k = k_begin, k_begin + step, ... (count columns),
-// each lane holding rows lane + 64 m.  The loads of the next kPf columns are
-// in flight while a column is used, so a wave waits for HBM once per kPf
-// columns instead of once per column.  want(i, k) masks the
-// rows a column contributes (the triangle), so masked lanes load nothing.
-constexpr int kPf = 4;  // measured best of 2, 4, 6, 8, 16 on 2048 fiber blocks
+// each lane holding rows lane + 64 m.  Columns come in rounds of kPf, in two
+// register sets that trade roles: the next round loads into one set while
+// the other's columns are used, so a wave waits for HBM once per round and
+// never for the loads it has just issued.  (A single ring reloaded in place
+// let the compiler hoist each reload above the mask that reads the old
+// value, copy the loaded value back at the end of the round, and wait there
+// for the round's own loads.)  want(i, k) masks the rows a column
+// contributes (the triangle).
+constexpr int kPf = 4;  // 6 or 8 take more than 256 VGPRs: one wave per SIMD instead of two
 
 // The loads are unconditional (row and column indices clamped into the
 // block; rows past n and columns past the stream are loaded and discarded)
@@ -568,48 +574,75 @@ constexpr int kPf = 4;  // measured best of 2, 4, 6, 8, 16 on 2048 fiber blocks
 template <typename T, typename Want, typename Use>
 __device__ __forceinline__ void stream_cols(const T* __restrict__ M, int n, int k_begin, int step,
                                             int count, int lane, Want&& want, Use&& use) {
-  T ring[kPf][kSlots];
-  auto load = [&](int idx, T (&v)[kSlots]) {
-    int k = k_begin + idx * step;
-    k = k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
-    const T* __restrict__ col = M + (long)k * n;
+  T ra[kPf][kSlots], rb[kPf][kSlots];
+  auto load = [&](int idx0, T (&r)[kPf][kSlots]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int m = 0; m < kSlots; ++m) {
-      const int i = 64 * m + lane;
-      v[m] = col[i < n ? i : n - 1];
+    for (int u = 0; u < kPf; ++u) {
+      int k = k_begin + (idx0 + u) * step;
+      k = k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
+      const T* __restrict__ col = M + (long)k * n;
+#pragma unroll
+      for (int m = 0; m < kSlots; ++m) {
+        const int i = 64 * m + lane;
+        r[u][m] = col[i < n ? i : n - 1];
+      }
     }
   };
-  auto take = [&](int idx, T (&slot)[kSlots]) __attribute__((always_inline)) {
-    const int k = k_begin + idx * step;
-    T v[kSlots];
+  // tail: stop at column index count (the whole rounds have no branch, so
+  // their loads stay where they are issued)
+  auto take = [&](auto tail, int idx0, const T (&r)[kPf][kSlots]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int m = 0; m < kSlots; ++m) {
-      const int i = 64 * m + lane;
-      v[m] = (i < n && want(i, k)) ? slot[m] : (T)0;
+    for (int u = 0; u < kPf; ++u) {
+      if constexpr (decltype(tail)::value) {
+        if (idx0 + u >= count) break;
+      }
+      const int k = k_begin + (idx0 + u) * step;
+      T v[kSlots];
+#pragma unroll
+      for (int m = 0; m < kSlots; ++m) {
+        const int i = 64 * m + lane;
+        v[m] = (i < n && want(i, k)) ? r[u][m] : (T)0;
+      }
+      use(k, v);
     }
-    load(idx + kPf, slot);
-    use(k, v);
   };
-#pragma unroll
-  for (int u = 0; u < kPf; ++u) load(u, ring[u]);
-  // whole rounds without a branch between the loads and their uses
+  load(0, ra);
+  // whole pairs of rounds without a branch between the loads and their uses
   int i0 = 0;
-  for (; i0 + kPf <= count; i0 += kPf) {
-#pragma unroll
-    for (int u = 0; u < kPf; ++u) take(i0 + u, ring[u]);
+  for (; i0 + 2 * kPf <= count; i0 += 2 * kPf) {
+    load(i0 + kPf, rb);
+    // The column index of each round passes through an empty asm placed
+    // after the loads: the round's work depends on it, so the compiler can
+    // neither sink the loads to their uses nor hoist the uses to the loads.
+    int ja = i0;
+    asm volatile("" : "+s"(ja) : : "memory");
+    take(std::false_type{}, ja, ra);
+    load(i0 + 2 * kPf, ra);
+    int jb = i0 + kPf;
+    asm volatile("" : "+s"(jb) : : "memory");
+    take(std::false_type{}, jb, rb);
   }
-#pragma unroll
-  for (int u = 0; u < kPf; ++u)
-    if (i0 + u < count) take(i0 + u, ring[u]);
+  if (i0 < count) {
+    load(i0 + kPf, rb);
+    take(std::true_type{}, i0, ra);
+    take(std::true_type{}, i0 + kPf, rb);
+  }
 }
 
-// w[slot] of lane (k % 64), slot k / 64 (k wave-uniform)
-__device__ __forceinline__ double pick_row(const double (&w)[kSlots], int k) {
-  const int ms = k >> 6;
-  double v = w[0];
+// v[slot] of lane (k % 64), slot k / 64 (k wave-uniform): one readlane per
+// slot and scalar selects.  (A select chain over the slots folds into a
+// dynamic index of v, which the compiler places in LDS: a write, a read and a
+// wait in the substitution's dependent chain.)
+template <typename T>
+__device__ __forceinline__ double pick_row(const T (&v)[kSlots], int k) {
+  const int ms = k >> 6, l = k & 63;
+  double r = readlane_f64((double)v[0], l);
 #pragma unroll
-  for (int m = 1; m < kSlots; ++m) v = (ms == m) ? w[m] : v;
-  return readlane_f64(v, k & 63);
+  for (int m = 1; m < kSlots; ++m) {
+    const double t = readlane_f64((double)v[m], l);
+    r = ms == m ? t : r;
+  }
+  return r;
 }
 
 // In-register LU solve of one block held by one wave: w <- U^-1 L^-1 w (the
@@ -626,11 +659,7 @@ __device__ __forceinline__ void trsv_lu(const T* __restrict__ L, int n, int lane
   // backward: w_k /= U(k, k); w_i -= U(i, k) w_k for i < k
   stream_cols(L, n, n - 1, -1, n, lane, [](int i, int k) { return i <= k; },
               [&](int k, const T (&c)[kSlots]) {
-                const int ms = k >> 6;
-                double d = (double)c[0];
-#pragma unroll
-                for (int m = 1; m < kSlots; ++m) d = (ms == m) ? (double)c[m] : d;
-                const double ukk = readlane_f64(d, k & 63);
+                const double ukk = pick_row(c, k);
                 const double wk = pick_row(w, k) / ukk;
 #pragma unroll
                 for (int m = 0; m < kSlots; ++m) {
