@@ -69,12 +69,14 @@ def test_mlp_training_reduces_loss():
 
 
 @pytest.mark.parametrize("N", [128, 256])
-@pytest.mark.parametrize("M,K", [(4099, 256), (1000, 48), (257, 160)])
+@pytest.mark.parametrize("M,K", [(4099, 256), (1000, 48), (257, 160), (4096, 256), (32832, 64)])
 @pytest.mark.parametrize("epi", ["store", "bias_relu", "bias", "relu_grad"])
 def test_per_node_layer_kernels_agree(N, M, K, epi):
     """The per-node layer kernels (resident-weight, streaming, tiled; chosen
     per N through f3d_gemm_set_stream) on one problem with each epilogue,
-    partial last row blocks included: each meets the FP64 product of the
+    partial last row blocks included, and whole row blocks only (M = 4096:
+    fewer row blocks than workgroups; M = 32832: 1026 or 513 row blocks, an
+    odd count for some workgroups): each meets the FP64 product of the
     BF16-rounded inputs, and the Z output of the bias + ReLU epilogue holds
     the pre-activation."""
     from fiber3d_amd import _lib as L
