@@ -471,14 +471,29 @@ def mlp_leg(reps=20):
 
 
 def copy_probe(mib):
+    """Three device copies of `mib` MiB (the calibration of the PMC byte
+    counters), then 20 more timed with HIP events: the read + write rate of
+    a plain streaming copy, the practical ceiling of kernels that read one
+    array and write another of the same size."""
     import torch
     a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 2 * a.numel() * 4
     print(json.dumps({"copy_probe_bytes_per_launch": {"read": a.numel() * 4,
-                                                      "write": a.numel() * 4}, "launches": 3}))
+                                                      "write": a.numel() * 4},
+                      "launches": 3 + reps, "timed_ms": round(ms, 4),
+                      "read_write_gbs": round(nbytes / ms / 1e6, 1)}))
 
 
 def main():
